@@ -1,0 +1,224 @@
+/* xtrack_amd.h -- C ABI of the MI355X multi-turn tracker (libxtrack_amd.so).
+ *
+ * The library replaces the generated `track_line` kernel of the reference
+ * (xtrack/tracker.py:546-564, compiled and launched by xobjects) with a
+ * persistent HIP kernel for gfx950.  Element parameters arrive as a raw
+ * element table whose per-type field layout mirrors the reference's
+ * element xofields (files under xtrack/beam_elements/); particle data arrive as the
+ * reference's SoA Particles layout (xtrack/particles/particles.py:26-83)
+ * given as plain device pointers.
+ *
+ * Every function returns 0 on success and a negative status on error;
+ * xtm_last_error() returns a message for the calling thread's last error.
+ * Nothing in this header depends on torch or on HIP types (streams are
+ * passed as void*; NULL means the default stream).
+ */
+#ifndef XTRACK_AMD_H
+#define XTRACK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XTM_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+#define XTM_OK 0
+#define XTM_ERR_ARG (-1)        /* invalid argument                          */
+#define XTM_ERR_UNSUPPORTED (-2)/* element/model/flag not supported          */
+#define XTM_ERR_HIP (-3)        /* HIP runtime error                         */
+#define XTM_ERR_NOMEM (-4)      /* allocation failed                         */
+
+/* ---- element type ids (raw table) -----------------------------------------
+ * Statically thick classes (those whose class declares isthick = True:
+ * Drift, Quadrupole, Bend, Sextupole, Octupole, Cavity) get the global
+ * |x|,|y| <= 1 m check after tracking (xtrack/tracker.py:681-689). */
+#define XTM_MARKER 0
+#define XTM_DRIFT 1
+#define XTM_MULTIPOLE 2
+#define XTM_QUADRUPOLE 3
+#define XTM_BEND 4
+#define XTM_SEXTUPOLE 5
+#define XTM_OCTUPOLE 6
+#define XTM_CAVITY 7
+#define XTM_LIMITRECT 8
+#define XTM_LIMITELLIPSE 9
+#define XTM_NUM_TYPES 10
+
+/* ---- raw parameter layout ---------------------------------------------------
+ * Element i occupies params[offsets[i] ...].  Integer fields are stored as
+ * doubles (exact for |v| < 2^53).
+ *
+ * DRIFT (xtrack/beam_elements/drift.py:31-34) */
+#define XTM_D_LENGTH 0
+#define XTM_D_MODEL 1
+#define XTM_D_SIZE 2
+
+/* Magnets: MULTIPOLE, QUADRUPOLE, BEND, SEXTUPOLE, OCTUPOLE share one layout.
+ * Fields a class does not have are 0.  Field meaning follows
+ * xtrack/beam_elements/{multipole,quadrupole,bend,sextupole,octupole}.py and
+ * _common.py:559-600.  After XTM_M_ARRAYS come knl[order+1], ksl[order+1],
+ * knl_rel[nrel], ksl_rel[nrel]. */
+#define XTM_M_LENGTH 0
+#define XTM_M_ORDER 1
+#define XTM_M_INV_FACTORIAL_ORDER 2
+#define XTM_M_NREL 3            /* len(knl_rel) == len(ksl_rel) >= 1          */
+#define XTM_M_MODEL 4
+#define XTM_M_INTEGRATOR 5
+#define XTM_M_NUM_MULTIPOLE_KICKS 6
+#define XTM_M_MAIN_IS_SKEW 7
+#define XTM_M_MAIN_ORDER 8      /* Multipole only                             */
+#define XTM_M_ISTHICK 9         /* Multipole only                             */
+#define XTM_M_HXL 10            /* Multipole only                             */
+#define XTM_M_K0 11
+#define XTM_M_K1 12
+#define XTM_M_K2 13
+#define XTM_M_K3 14
+#define XTM_M_K0S 15
+#define XTM_M_K1S 16
+#define XTM_M_K2S 17
+#define XTM_M_K3S 18
+#define XTM_M_H 19              /* Bend only (angle / length)                 */
+#define XTM_M_EDGE_ENTRY_ACTIVE 20
+#define XTM_M_EDGE_EXIT_ACTIVE 21
+#define XTM_M_EDGE_ENTRY_MODEL 22
+#define XTM_M_EDGE_EXIT_MODEL 23
+#define XTM_M_EDGE_ENTRY_ANGLE 24
+#define XTM_M_EDGE_EXIT_ANGLE 25
+#define XTM_M_EDGE_ENTRY_ANGLE_FDOWN 26
+#define XTM_M_EDGE_EXIT_ANGLE_FDOWN 27
+#define XTM_M_EDGE_ENTRY_FINT 28
+#define XTM_M_EDGE_EXIT_FINT 29
+#define XTM_M_EDGE_ENTRY_HGAP 30
+#define XTM_M_EDGE_EXIT_HGAP 31
+#define XTM_M_DELTA_TAPER 32
+#define XTM_M_RADIATION_FLAG 33
+#define XTM_M_ARRAYS 34
+
+/* CAVITY (xtrack/beam_elements/cavity.py:63-76) */
+#define XTM_C_LENGTH 0
+#define XTM_C_VOLTAGE 1
+#define XTM_C_FREQUENCY 2
+#define XTM_C_HARMONIC 3
+#define XTM_C_LAG 4
+#define XTM_C_PHASE 5
+#define XTM_C_ABSOLUTE_TIME 6
+#define XTM_C_NUM_KICKS 7
+#define XTM_C_MODEL 8
+#define XTM_C_INTEGRATOR 9
+#define XTM_C_LAG_TAPER 10
+#define XTM_C_PHASE_TAPER 11
+#define XTM_C_SIZE 12
+
+/* LIMITRECT (limit_rect.py), LIMITELLIPSE (limit_ellipse.py:25-29) */
+#define XTM_LR_MIN_X 0
+#define XTM_LR_MAX_X 1
+#define XTM_LR_MIN_Y 2
+#define XTM_LR_MAX_Y 3
+#define XTM_LR_SIZE 4
+#define XTM_LE_A_SQU 0
+#define XTM_LE_B_SQU 1
+#define XTM_LE_A_B_SQU 2
+#define XTM_LE_SIZE 3
+
+/* ---- track flags (xtrack/track_flags.py:5-12) ------------------------------ */
+#define XTM_FLAG_BACKTRACK 0
+#define XTM_FLAG_KILL_CAVITY_KICK 2
+#define XTM_FLAG_IGNORE_GLOBAL_APERTURE 3
+#define XTM_FLAG_IGNORE_LOCAL_APERTURE 4
+#define XTM_FLAG_SR_TAPER 5
+
+/* ---- particles: SoA, one pointer per per-particle field --------------------
+ * Mirrors ParticlesData (xtrack/particles/particles.py:26-83).  All pointers
+ * refer to arrays of length `capacity` in the memory space the consumer
+ * expects (device memory for xtm_track_line).  Unused slots carry
+ * state = -999999999. */
+typedef struct xtm_particles {
+    int64_t capacity;
+    double q0;
+    double mass0;
+    double t_sim;
+    double *p0c, *gamma0, *beta0, *s, *zeta, *x, *y, *px, *py;
+    double *ptau, *delta, *rpp, *rvv;
+    double *chi, *charge_ratio, *weight, *ax, *ay;
+    double *spin_x, *spin_y, *spin_z, *anomalous_magnetic_moment;
+    int64_t *pdg_id, *particle_id, *at_element, *at_turn, *state;
+    int64_t *parent_particle_id;
+    uint32_t *rng_s1, *rng_s2, *rng_s3, *rng_s4;
+} xtm_particles;
+
+/* ---- turn-by-turn monitor (xtrack/monitors/particles_monitor.py:190-203) --
+ * `data` holds n_records slots per field; record index for particle id p and
+ * turn t (n_repetitions == 1) is (stop-start)*(p - part_id_start) + t - start
+ * (xtrack/monitors/particles_monitor.h:36-44). */
+typedef struct xtm_monitor {
+    int64_t start_at_turn;
+    int64_t stop_at_turn;
+    int64_t part_id_start;
+    int64_t part_id_end;
+    int64_t ebe_mode;
+    int64_t n_records;
+    int64_t n_repetitions;
+    int64_t repetition_period;
+    xtm_particles data;
+} xtm_monitor;
+
+/* Opaque device-resident element table for one line. */
+typedef struct xtm_line xtm_line;
+
+/* Library ABI version (XTM_ABI_VERSION). */
+int32_t xtm_abi_version(void);
+
+/* Message describing the last error of the calling thread ("" if none). */
+const char *xtm_last_error(void);
+
+/* Number of visible HIP devices (>= 0) or a negative status. */
+int32_t xtm_device_count(void);
+
+/* Build the device element table of a line.  Replaces TrackerData's element
+ * buffer and ElementRefData (xtrack/tracker_data.py:67-172, 237-255) and the
+ * per-class parameter derivation done at kernel entry (e.g.
+ * elements_src/quadrupole.h:15-75).  `device` is the HIP device ordinal.
+ * The arrays are read once; the caller keeps ownership. */
+int32_t xtm_line_create(const int32_t *types, const int64_t *offsets,
+                        const double *params, int64_t num_elements,
+                        double line_length, int32_t device, xtm_line **out);
+
+/* Release a line and its device memory. */
+int32_t xtm_line_destroy(xtm_line *line);
+
+/* Number of elements / distinct parameter records of a created line. */
+int64_t xtm_line_num_elements(const xtm_line *line);
+int64_t xtm_line_num_records(const xtm_line *line);
+
+/* Track particles in place, asynchronously on `stream`.  Same arguments and
+ * semantics as the reference kernel `track_line` (xtrack/tracker.py:546-564):
+ * turns [0, num_turns), elements [ele_start, ele_start+num_ele_track) each
+ * turn, end-of-turn actions when flag_end_turn_actions > 0, monitor mode
+ * flag_monitor (0 none, 1 turn-by-turn, 2 element-by-element).  `monitor`
+ * may be NULL when flag_monitor == 0.  Particle and monitor pointers are
+ * device pointers. */
+int32_t xtm_track_line(xtm_line *line, const xtm_particles *particles,
+                       int32_t num_turns, int32_t ele_start,
+                       int32_t num_ele_track, int32_t flag_end_turn_actions,
+                       int32_t flag_reset_s_at_end_turn, int32_t flag_monitor,
+                       const xtm_monitor *monitor, uint64_t track_flags,
+                       void *stream);
+
+/* Wait for all work queued on `stream`. */
+int32_t xtm_synchronize(void *stream);
+
+/* Diagnostics: evaluate the library's device elementary functions
+ * (fn: 0 sin, 1 cos, 2 sinh, 3 cosh, 4 exp, 5 tan, 6 sqrt, 7 div by x[i+n])
+ * on n device doubles.  No counterpart in the reference; used to check that
+ * the device and host builds of the functions agree bit for bit. */
+int32_t xtm_math_eval(int32_t fn, const double *x, double *y, int64_t n,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XTRACK_AMD_H */

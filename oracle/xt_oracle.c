@@ -1,0 +1,1123 @@
+/* xt_oracle.c -- CPU restatement of the reference tracker for the parity
+ * tests.  TEST INFRASTRUCTURE ONLY: imported by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the
+ * product path.
+ *
+ * It restates, in plain C, the OpenMP-context behaviour of the generated
+ * `track_line` kernel (xtrack/tracker.py:544-751) and the element code it
+ * calls, keeping the reference's expression order statement by statement
+ * (build with -ffp-contract=off, no fast-math):
+ *   loop/bookkeeping   tracker.py:566-748, headers/track.h:33-50,
+ *                      particles/local_particle_custom_api.h
+ *   Drift              beam_elements/elements_src/drift.h, track_drift.h
+ *   magnets            multipole.h, quadrupole.h, bend.h, sextupole.h,
+ *                      octupole.h -> track_magnet.h, track_magnet_kick.h,
+ *                      track_magnet_drift.h, track_magnet_configure.h,
+ *                      track_magnet_edge.h (linear model),
+ *                      track_dipole_edge_linear.h, headers/factorial.h
+ *   Cavity             cavity.h, track_rf.h
+ *   apertures          limitrect.h, limitellipse.h
+ *   monitor            monitors/particles_monitor.h
+ * Compiled with XTO_USE_LIBM the elementary functions are the platform libm
+ * (what the reference's ContextCpu calls); otherwise the project library
+ * xt_math.h is used, which is what the GPU kernel evaluates.
+ *
+ * Not restated (rejected with XTM_ERR_UNSUPPORTED): radiation and spin
+ * (compiled out by XTRACK_MULTIPOLE_NO_SYNRAD, the default line config,
+ * xtrack/line.py:284), full/dipole-only edge models, the bend-kick-bend
+ * model, rbends, solenoids, misalignments, the multi-element monitor and
+ * the SR-taper flag.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/xtrack_amd.h"
+
+#ifdef XTO_USE_LIBM
+#define M_SIN sin
+#define M_COS cos
+#define M_SINH sinh
+#define M_COSH cosh
+#define M_TAN tan
+#define M_EXP exp
+#else
+#include "../xtrack_amd/csrc/xt_math.h"
+#define M_SIN xt_sin
+#define M_COS xt_cos
+#define M_SINH xt_sinh
+#define M_COSH xt_cosh
+#define M_TAN xt_tan
+#define M_EXP xt_exp
+#endif
+
+/* headers/constants.h */
+#define C_LIGHT (299792458.0)
+#define PI (3.1415926535897932384626433832795028841971693993751)
+#define DEG2RAD (0.0174532925199432957692369076848861271344287188854)
+#define XTRACK_GLOBAL_XY_LIMIT 1.0     /* xtrack/line.py:287 */
+#define LAST_INVALID_STATE (-999999999)
+
+#define NONZERO(x) ((x) != 0.0)
+#define POW2(x) ((x) * (x))
+
+/* ---------------------------------------------------------------------------
+ * LocalParticle: pointer to the SoA plus the index range of a chunk
+ * (particles.py:96-109). */
+typedef struct {
+    const xtm_particles *P;
+    int64_t ipart;
+    int64_t endpart;
+    uint64_t track_flags;
+    double line_length;
+} LP;
+
+#define GET(f) (part->P->f[part->ipart])
+#define SET(f, v) (part->P->f[part->ipart] = (v))
+#define ADD(f, v) (part->P->f[part->ipart] += (v))
+
+static int check_flag(const LP *part, int idx) { return (int)((part->track_flags >> idx) & 1); }
+
+/* START_PER_PARTICLE_BLOCK for the OpenMP context (headers/track.h:33-50):
+   iterate the chunk, skipping particles with state <= 0. */
+#define FOR_ACTIVE(part0, part)                                              \
+    for (int64_t _ii = (part0)->ipart; _ii < (part0)->endpart; _ii++) {     \
+        LP _lp = *(part0); LP *part = &_lp; part->ipart = _ii;              \
+        if (part->P->state[_ii] > 0)
+#define END_FOR_ACTIVE }
+
+/* --------------------------------------------------------------------------
+ * local_particle_custom_api.h */
+static void update_ptau(LP *part, double new_ptau_value) {     /* :21-32 */
+    double const beta0 = GET(beta0);
+    double const ptau = new_ptau_value;
+    double const irpp = sqrt(ptau * ptau + 2 * ptau / beta0 + 1);
+    double const new_rpp = 1. / irpp;
+    double const new_rvv = irpp / (1 + beta0 * ptau);
+    SET(delta, irpp - 1);
+    SET(rvv, new_rvv);
+    SET(ptau, ptau);
+    SET(rpp, new_rpp);
+}
+
+static void add_to_energy(LP *part, double delta_energy, int pz_only) { /* :196-216 */
+    double ptau = GET(ptau);
+    double const p0c = GET(p0c);
+    double const charge_ratio = GET(charge_ratio);
+    double const chi = GET(chi);
+    double const mass_ratio = charge_ratio / chi;
+    ptau += delta_energy / p0c / mass_ratio;
+    double const old_rpp = GET(rpp);
+    update_ptau(part, ptau);
+    if (!pz_only) {
+        double const new_rpp = GET(rpp);
+        double const f = old_rpp / new_rpp;
+        part->P->px[part->ipart] *= f;
+        part->P->py[part->ipart] *= f;
+    }
+}
+
+static void exchange(const xtm_particles *P, int64_t i1, int64_t i2) {  /* particles.py:360-374 */
+#define XCH(T, f) { T t = P->f[i2]; P->f[i2] = P->f[i1]; P->f[i1] = t; }
+    XCH(double, p0c) XCH(double, gamma0) XCH(double, beta0) XCH(double, s)
+    XCH(double, zeta) XCH(double, x) XCH(double, y) XCH(double, px) XCH(double, py)
+    XCH(double, ptau) XCH(double, delta) XCH(double, rpp) XCH(double, rvv)
+    XCH(double, chi) XCH(double, charge_ratio) XCH(double, weight) XCH(double, ax)
+    XCH(double, ay) XCH(double, spin_x) XCH(double, spin_y) XCH(double, spin_z)
+    XCH(double, anomalous_magnetic_moment)
+    XCH(int64_t, pdg_id) XCH(int64_t, particle_id) XCH(int64_t, at_element)
+    XCH(int64_t, at_turn) XCH(int64_t, state) XCH(int64_t, parent_particle_id)
+    XCH(uint32_t, rng_s1) XCH(uint32_t, rng_s2) XCH(uint32_t, rng_s3) XCH(uint32_t, rng_s4)
+#undef XCH
+}
+
+/* check_is_active, OpenMP variant (custom_api.h:131-162): partition the
+   chunk so active particles come first; report whether any is alive. */
+static int64_t check_is_active(LP *part) {
+    int64_t left = part->ipart;
+    int64_t right = part->endpart - 1;
+    int64_t swap_made = 0, has_alive = 0;
+    int64_t *st = part->P->state;
+    if (left == right) return st[left] > 0;
+    while (left < right) {
+        if (st[left] > 0) { left++; has_alive = 1; }
+        else if (st[right] <= 0) right--;
+        else { exchange(part->P, left, right); left++; right--; swap_made = 1; }
+    }
+    return swap_made || has_alive;
+}
+
+static void count_reorganized(const xtm_particles *P, int64_t ipart, int64_t endpart,
+                              int64_t *n_active, int64_t *n_lost) {      /* :165-182 */
+    int64_t na = 0, nl = 0;
+    for (int64_t i = ipart; i < endpart; i++) {
+        if (P->state[i] <= LAST_INVALID_STATE) break;
+        else if (P->state[i] > 0) na++;
+        else nl++;
+    }
+    *n_active = na; *n_lost = nl;
+}
+
+static void global_aperture_check(LP *part0) {                  /* :262-285 */
+    if (check_flag(part0, XTM_FLAG_IGNORE_GLOBAL_APERTURE)) return;
+    FOR_ACTIVE(part0, part) {
+        double const x = GET(x);
+        double const y = GET(y);
+        int64_t const is_within = (int64_t)((x >= -XTRACK_GLOBAL_XY_LIMIT) &&
+                                            (x <= XTRACK_GLOBAL_XY_LIMIT) &&
+                                            (y >= -XTRACK_GLOBAL_XY_LIMIT) &&
+                                            (y <= XTRACK_GLOBAL_XY_LIMIT));
+        if (GET(state) > 0 && !is_within) SET(state, -1);
+    } END_FOR_ACTIVE
+}
+
+/* --------------------------------------------------------------------------
+ * headers/factorial.h: 1/n! for n < 20 (correctly rounded), tgamma above. */
+static double one_over_factorial(int64_t n) {
+    static const double t[20] = {
+        0x1p+0, 0x1p+0, 0x1p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+        0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+        0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+        0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
+        0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45,
+        0x1.952c77030ad4ap-49, 0x1.6827863b97d97p-53, 0x1.2f49b46814157p-57};
+    if (n < 0) return 0.;
+    if (n < 20) return t[n];
+    return 1. / tgamma((double)(n + 1));
+}
+
+/* --------------------------------------------------------------------------
+ * Drift maps (track_drift.h:11-41, track_magnet_drift.h:13-42) */
+static void drift_expanded(LP *part, double length) {
+    double const rpp = GET(rpp);
+    double const rv0v = 1. / GET(rvv);
+    double const xp = GET(px) * rpp;
+    double const yp = GET(py) * rpp;
+    double const dzeta = 1 - rv0v * (1. + (xp * xp + yp * yp) / 2.);
+    ADD(x, xp * length);
+    ADD(y, yp * length);
+    ADD(s, length);
+    ADD(zeta, length * dzeta);
+}
+
+static void drift_exact(LP *part, double length) {
+    double const px = GET(px);
+    double const py = GET(py);
+    double const rv0v = 1. / GET(rvv);
+    double const one_plus_delta = 1. + GET(delta);
+    double const one_over_pz = 1. / sqrt(one_plus_delta * one_plus_delta - px * px - py * py);
+    double const dzeta = 1 - rv0v * one_plus_delta * one_over_pz;
+    ADD(x, px * one_over_pz * length);
+    ADD(y, py * one_over_pz * length);
+    ADD(zeta, dzeta * length);
+    ADD(s, length);
+}
+
+/* track_magnet_drift.h:45-87 */
+static void polar_drift(LP *part, const double length, const double h) {
+    const double rvv = GET(rvv);
+    const double x = GET(x);
+    const double y = GET(y);
+    const double px = GET(px);
+    const double py = GET(py);
+    const double s = length;
+    const double one_plus_delta = GET(delta) + 1.0;
+    const double pz = sqrt(POW2(one_plus_delta) - POW2(px) - POW2(py));
+    double const rho = 1 / h;
+    const double ca = M_COS(h * s);
+    const double sa = M_SIN(h * s);
+    const double sa2 = M_SIN(0.5 * h * s);
+    const double _pz = 1 / pz;
+    const double pxt = px * _pz;
+    const double _ptt = 1 / (ca - sa * pxt);
+    const double pst = (x + rho) * sa * _pz * _ptt;
+    double new_x = (x + rho * (2 * sa2 * sa2 + sa * pxt)) * _ptt;
+    double new_px = ca * px + sa * pz;
+    double new_y = y + pst * py;
+    double delta_ell = one_plus_delta * (x + rho) * sa / ca / pz / (1 - px * sa / ca / pz);
+    SET(x, new_x);
+    SET(px, new_px);
+    SET(y, new_y);
+    ADD(zeta, length - delta_ell / rvv);
+    ADD(s, s);
+}
+
+/* track_magnet_drift.h:91-213 */
+static void expanded_dipole_quad(LP *part, const double length, const double k0_,
+                                 const double k1_, const double h) {
+    const double x = GET(x);
+    const double y = GET(y);
+    const double px = GET(px);
+    const double py = GET(py);
+    const double rvv = GET(rvv);
+    const double delta_plus_1 = GET(delta) + 1;
+    const double chi = GET(chi);
+    const double k0 = chi * k0_ / delta_plus_1;
+    const double k1 = chi * k1_ / delta_plus_1;
+    const double Kx = k0 * h + k1;
+    const double Ky = -k1;
+    double x_, px_, y_, py_, Sx, Sy, Cx, Cy;
+    if (Kx > 0.0) {
+        double sqrt_Kx = sqrt(Kx);
+        Sx = M_SIN(sqrt_Kx * length) / sqrt_Kx;
+        Cx = M_COS(sqrt_Kx * length);
+    } else if (Kx < 0.0) {
+        double sqrt_Kx = sqrt(-Kx);
+        Sx = M_SINH(sqrt_Kx * length) / sqrt_Kx;
+        Cx = M_COSH(sqrt_Kx * length);
+    } else {
+        Sx = length;
+        Cx = 1.0;
+    }
+    if (Ky > 0.0) {
+        double sqrt_Ky = sqrt(Ky);
+        Sy = M_SIN(sqrt_Ky * length) / sqrt_Ky;
+        Cy = M_COS(sqrt_Ky * length);
+    } else if (Ky < 0.0) {
+        double sqrt_Ky = sqrt(-Ky);
+        Sy = M_SINH(sqrt_Ky * length) / sqrt_Ky;
+        Cy = M_COSH(sqrt_Ky * length);
+    } else {
+        Sy = length;
+        Cy = 1.0;
+    }
+    const double xp = px / delta_plus_1;
+    const double yp = py / delta_plus_1;
+    const double A = -Kx * x - k0 + h;
+    const double B = xp;
+    const double C = -Ky * y;
+    const double D = yp;
+    x_ = x * Cx + xp * Sx;
+    y_ = y * Cy + yp * Sy;
+    px_ = (A * Sx + B * Cx) * delta_plus_1;
+    py_ = (C * Sy + D * Cy) * delta_plus_1;
+    if (NONZERO(Kx))
+        x_ = x_ + (k0 - h) * (Cx - 1.0) / Kx;
+    else
+        x_ = x_ - (k0 - h) * 0.5 * POW2(length);
+    double length_ = length;
+    if (NONZERO(Kx)) {
+        length_ -= (h * ((Cx - 1.0) * xp + Sx * A + length * (k0 - h))) / Kx;
+        length_ += 0.5 * (
+            - (POW2(A) * Cx * Sx) / (2.0 * Kx)
+            + (POW2(B) * Cx * Sx) / 2.0
+            + (POW2(A) * length) / (2.0 * Kx)
+            + (POW2(B) * length) / 2.0
+            - (A * B * POW2(Cx)) / Kx
+            + (A * B) / Kx);
+    } else {
+        length_ += h * length * (3.0 * length * xp + 6.0 * x - (k0 - h) * POW2(length)) / 6.0;
+        length_ += 0.5 * (POW2(B)) * length;
+    }
+    if (NONZERO(Ky)) {
+        length_ += 0.5 * (
+            - (POW2(C) * Cy * Sy) / (2.0 * Ky)
+            + (POW2(D) * Cy * Sy) / 2.0
+            + (POW2(C) * length) / (2.0 * Ky)
+            + (POW2(D) * length) / 2.0
+            - (C * D * POW2(Cy)) / Ky
+            + (C * D) / Ky);
+    } else {
+        length_ += 0.5 * POW2(D) * length;
+    }
+    const double dzeta = length - length_ / rvv;
+    SET(x, x_);
+    SET(px, px_);
+    SET(y, y_);
+    SET(py, py_);
+    ADD(zeta, dzeta);
+    ADD(s, length);
+}
+
+/* track_magnet_drift.h:468-555 (models 0,1,2,3,7,8) */
+static void magnet_drift(LP *part, const double length, const double k0, const double k1,
+                         const double h, const int64_t drift_model) {
+    if (drift_model == -1) return;
+    if (length == 0.0) return;
+    switch (drift_model) {
+    case 0: drift_expanded(part, length); break;
+    case 1: drift_exact(part, length); break;
+    case 2: polar_drift(part, length, h); break;
+    case 3: expanded_dipole_quad(part, length, k0, k1, h); break;
+    case 7:
+        polar_drift(part, 0.6756035959798289 * length, h);
+        SET(px, GET(px) - 1.3512071919596578 * k0 * GET(chi) * length);
+        polar_drift(part, -0.17560359597982889 * length, h);
+        SET(px, GET(px) - (-1.7024143839193155) * k0 * GET(chi) * length);
+        polar_drift(part, -0.17560359597982889 * length, h);
+        SET(px, GET(px) - 1.3512071919596578 * k0 * GET(chi) * length);
+        polar_drift(part, 0.6756035959798289 * length, h);
+        break;
+    case 8:
+        polar_drift(part, 3.922568052387799819591407413100e-01 * length, h);
+        SET(px, GET(px) - 7.845136104775599639182814826199e-01 * k0 * GET(chi) * length);
+        polar_drift(part, 5.100434119184584780271052295575e-01 * length, h);
+        SET(px, GET(px) - 2.355732133593569921359289764951e-01 * k0 * GET(chi) * length);
+        polar_drift(part, -4.710533854097565531482416645304e-01 * length, h);
+        SET(px, GET(px) - (-1.177679984178870098432412305556e+00) * k0 * GET(chi) * length);
+        polar_drift(part, 6.875316825251809316199569366290e-02 * length, h);
+        SET(px, GET(px) - 1.315186320683906284756403692882e+00 * k0 * GET(chi) * length);
+        polar_drift(part, 6.875316825251809316199569366290e-02 * length, h);
+        SET(px, GET(px) - (-1.177679984178870098432412305556e+00) * k0 * GET(chi) * length);
+        polar_drift(part, -4.710533854097565531482416645304e-01 * length, h);
+        SET(px, GET(px) - 2.355732133593569921359289764951e-01 * k0 * GET(chi) * length);
+        polar_drift(part, 5.100434119184584780271052295575e-01 * length, h);
+        SET(px, GET(px) - 7.845136104775599639182814826199e-01 * k0 * GET(chi) * length);
+        polar_drift(part, 3.922568052387799819591407413100e-01 * length, h);
+        break;
+    default: break;
+    }
+}
+
+/* --------------------------------------------------------------------------
+ * Kicks (track_magnet_kick.h) */
+static void kick_simple_coords(double const x, double const y, double const chi,
+                               int64_t order, double inv_factorial,
+                               const double *knl, const double *ksl, double factor,
+                               double kick_weight, double *dpx, double *dpy) {  /* :183-228 */
+    int64_t index = order;
+    double dpx_mul = chi * knl[index] * factor * inv_factorial;
+    double dpy_mul = chi * ksl[index] * factor * inv_factorial;
+    while (index > 0) {
+        double const zre = dpx_mul * x - dpy_mul * y;
+        double const zim = dpx_mul * y + dpy_mul * x;
+        inv_factorial *= index;
+        index -= 1;
+        double this_knl = chi * knl[index] * factor;
+        double this_ksl = chi * ksl[index] * factor;
+        dpx_mul = this_knl * inv_factorial + zre;
+        dpy_mul = this_ksl * inv_factorial + zim;
+    }
+    dpx_mul = -dpx_mul;
+    *dpx = kick_weight * dpx_mul;
+    *dpy = kick_weight * dpy_mul;
+}
+
+static void kick_simple(LP *part, int64_t order, double inv_factorial, const double *knl,
+                        const double *ksl, double factor, double kick_weight) {     /* :232-262 */
+    double const chi = GET(chi);
+    double const x = GET(x);
+    double const y = GET(y);
+    double dpx, dpy;
+    kick_simple_coords(x, y, chi, order, inv_factorial, knl, ksl, factor, kick_weight, &dpx, &dpy);
+    ADD(px, dpx);
+    ADD(py, dpy);
+}
+
+typedef struct {
+    double length;                 /* core length */
+    int64_t order; double inv_factorial_order; const double *knl, *ksl;
+    int64_t order_rel; double inv_factorial_order_rel; const double *knl_rel, *ksl_rel;
+    double rel_ref_strength;
+    double factor_knl_ksl;
+    int64_t num_multipole_kicks;
+    int kick_rot_frame;
+    int64_t drift_model;
+    int64_t integrator;
+    double k0_drift, k1_drift, ks_drift, h_drift;
+    double k0_kick, k1_kick, h_kick, hxl;
+    double k0_h_correction, k1_h_correction;
+    double k2, k3, k0s, k1s, k2s, k3s;
+} body_args;
+
+/* track_magnet_kick.h:24-144 */
+static void magnet_kick(LP *part, const body_args *a, double kick_weight) {
+    double const chi = GET(chi);
+    double const x = GET(x);
+    double const y = GET(y);
+    double const length = a->length;
+    double knl_main[4] = {a->k0_kick, a->k1_kick, a->k2, a->k3};
+    double ksl_main[4] = {a->k0s, a->k1s, a->k2s, a->k3s};
+    for (int index = 0; index < 4; index++) {
+        knl_main[index] = knl_main[index] * length;
+        ksl_main[index] = ksl_main[index] * length;
+    }
+    kick_simple(part, a->order, a->inv_factorial_order, a->knl, a->ksl, a->factor_knl_ksl, kick_weight);
+    kick_simple(part, a->order_rel, a->inv_factorial_order_rel, a->knl_rel, a->ksl_rel,
+                a->factor_knl_ksl * a->rel_ref_strength, kick_weight);
+    kick_simple(part, 3, 1. / (3 * 2), knl_main, ksl_main, 1, kick_weight);
+    double dpx = 0;
+    double dpy = 0;
+    double dzeta = 0;
+    double const h = a->h_kick;
+    if (a->kick_rot_frame) {
+        double const hl = h * length * kick_weight + a->hxl * kick_weight;
+        dpx += hl * (1. + GET(delta));
+        double const rv0v = 1. / GET(rvv);
+        dzeta += -rv0v * hl * x;
+    }
+    double htot = h;
+    if (length != 0) htot += a->hxl / length;
+    double k0l_mult = 0;
+    if (a->order >= 0) k0l_mult = a->knl[0] * a->factor_knl_ksl;
+    if (a->order_rel >= 0) k0l_mult += a->knl_rel[0] * a->factor_knl_ksl * a->rel_ref_strength;
+    dpx += -chi * (a->k0_h_correction * length + k0l_mult) * kick_weight * htot * x;
+    double k1l_mult = 0;
+    if (a->order >= 1) k1l_mult = a->knl[1] * a->factor_knl_ksl;
+    if (a->order_rel >= 1) k1l_mult += a->knl_rel[1] * a->factor_knl_ksl * a->rel_ref_strength;
+    dpx += htot * chi * (a->k1_h_correction * length + k1l_mult) * kick_weight * (-x * x + 0.5 * y * y);
+    dpy += htot * chi * (a->k1_h_correction * length + k1l_mult) * kick_weight * x * y;
+    ADD(px, dpx);
+    ADD(py, dpy);
+    ADD(zeta, dzeta);
+}
+
+/* track_magnet.h:26-285 with XTRACK_MULTIPOLE_NO_SYNRAD */
+static void magnet_body(LP *part, const body_args *a) {
+    const double length = a->length;
+    const int64_t nmk = a->num_multipole_kicks;
+#define MDRIFT(dl) magnet_drift(part, (dl), a->k0_drift, a->k1_drift, a->h_drift, a->drift_model)
+#define MKICK(w) magnet_kick(part, a, (w))
+    if (nmk == 0 && a->k0_kick == 0 && a->k1_kick == 0 && a->h_kick == 0) {
+        MDRIFT(length);
+        return;
+    }
+    if (a->integrator == 1) {                                        /* TEAPOT */
+        const double kick_weight = 1. / nmk;
+        double edge_drift_weight = 0.5;
+        double inside_drift_weight = 0;
+        if (nmk > 1) {
+            edge_drift_weight = 1. / (2 * (1 + nmk));
+            inside_drift_weight = (((double)nmk) / ((double)(nmk * nmk) - 1));
+        }
+        MDRIFT(edge_drift_weight * length);
+        for (int i_kick = 0; i_kick < nmk - 1; i_kick++) {
+            MKICK(kick_weight);
+            MDRIFT(inside_drift_weight * length);
+        }
+        MKICK(kick_weight);
+        MDRIFT(edge_drift_weight * length);
+    } else if (a->integrator == 3) {                                 /* uniform */
+        const double kick_weight = 1. / nmk;
+        const double drift_weight = kick_weight;
+        for (int i_kick = 0; i_kick < nmk; i_kick++) {
+            MDRIFT(0.5 * drift_weight * length);
+            MKICK(kick_weight);
+            MDRIFT(0.5 * drift_weight * length);
+        }
+    } else if (a->integrator == 2) {                                 /* Yoshida 4 */
+        const int64_t n_kicks_yoshida = 7;
+        const int64_t num_slices = (nmk / n_kicks_yoshida + (nmk % n_kicks_yoshida != 0));
+        const double slice_length = length / (num_slices);
+        const double kick_weight = 1. / num_slices;
+        const double d_yoshida[] = {3.922568052387799819591407413100e-01,
+                                    5.100434119184584780271052295575e-01,
+                                    -4.710533854097565531482416645304e-01,
+                                    6.875316825251809316199569366290e-02};
+        const double k_yoshida[] = {7.845136104775599639182814826199e-01,
+                                    2.355732133593569921359289764951e-01,
+                                    -1.177679984178870098432412305556e+00,
+                                    1.315186320683906284756403692882e+00};
+        for (int ii = 0; ii < num_slices; ii++) {
+            MDRIFT(slice_length * d_yoshida[0]);
+            MKICK(kick_weight * k_yoshida[0]);
+            MDRIFT(slice_length * d_yoshida[1]);
+            MKICK(kick_weight * k_yoshida[1]);
+            MDRIFT(slice_length * d_yoshida[2]);
+            MKICK(kick_weight * k_yoshida[2]);
+            MDRIFT(slice_length * d_yoshida[3]);
+            MKICK(kick_weight * k_yoshida[3]);
+            MDRIFT(slice_length * d_yoshida[3]);
+            MKICK(kick_weight * k_yoshida[2]);
+            MDRIFT(slice_length * d_yoshida[2]);
+            MKICK(kick_weight * k_yoshida[1]);
+            MDRIFT(slice_length * d_yoshida[1]);
+            MKICK(kick_weight * k_yoshida[0]);
+            MDRIFT(slice_length * d_yoshida[0]);
+        }
+    }
+#undef MDRIFT
+#undef MKICK
+}
+
+/* track_magnet_kick.h:149-180 */
+static int kick_is_inactive(int64_t order, const double *knl, const double *ksl, double k0,
+                            double k1, double k2, double k3, double k0s, double k1s,
+                            double k2s, double k3s, double h) {
+    if (h != 0) return 0;
+    if (k0 != 0) return 0;
+    if (k1 != 0) return 0;
+    if (k2 != 0) return 0;
+    if (k3 != 0) return 0;
+    if (k0s != 0) return 0;
+    if (k1s != 0) return 0;
+    if (k2s != 0) return 0;
+    if (k3s != 0) return 0;
+    for (int64_t index = order; index >= 0; index--) {
+        if (knl[index] != 0) return 0;
+        if (ksl[index] != 0) return 0;
+    }
+    return 1;
+}
+
+/* track_magnet_configure.h:7-195 (k0_drift .. kick_rot_frame) */
+static int configure_tracking_model(int64_t model, double k0, double k1, double h, body_args *a) {
+    if (model == 1) model = 3;
+    int h_is_zero = (fabs(h) < 1e-8);
+    int64_t dm;
+    if (model == 3) dm = h_is_zero ? 1 : 7;
+    else if (model == 4) dm = 3;
+    else if (model == 5) dm = 1;
+    else if (model == 6) dm = 0;
+    else if (model == -1) dm = -1;
+    else if (model == 7) dm = h_is_zero ? 1 : 2;
+    else if (model == 8) dm = h_is_zero ? 1 : 8;
+    else return XTM_ERR_UNSUPPORTED;  /* 2 bend-kick-bend, -2 solenoid */
+    a->ks_drift = 0.0;
+    if (dm == -1 || dm == 0 || dm == 1) {
+        a->k0_drift = 0.0; a->k1_drift = 0.0; a->h_drift = 0.0;
+        a->k0_kick = k0; a->k1_kick = k1; a->h_kick = h;
+        a->k0_h_correction = k0; a->k1_h_correction = k1; a->kick_rot_frame = 1;
+    } else if (dm == 2) {
+        a->k0_drift = 0.0; a->k1_drift = 0.0; a->h_drift = h;
+        a->k0_kick = k0; a->k1_kick = k1; a->h_kick = h;
+        a->k0_h_correction = k0; a->k1_h_correction = k1; a->kick_rot_frame = 0;
+    } else if (dm == 3) {
+        a->k0_drift = k0; a->k1_drift = k1; a->h_drift = h;
+        a->k0_kick = 0.0; a->k1_kick = 0.0; a->h_kick = h;
+        a->k0_h_correction = 0.; a->k1_h_correction = k1; a->kick_rot_frame = 0;
+    } else { /* 7, 8 */
+        a->k0_drift = k0; a->k1_drift = 0.0; a->h_drift = h;
+        a->k0_kick = 0.0; a->k1_kick = k1; a->h_kick = h;
+        a->k0_h_correction = k0; a->k1_h_correction = k1; a->kick_rot_frame = 0;
+    }
+    a->drift_model = dm;
+    return XTM_OK;
+}
+
+/* track_dipole_edge_linear.h + track_magnet_edge.h:28-80 (models 0, -1, 3) */
+static void magnet_edge(LP *part0, int64_t model, int is_exit, double half_gap,
+                        const double *knorm, double factor_knl_ksl, int64_t kl_order,
+                        const double *knl, const double *knl_rel, double factor_knl_ksl_rel,
+                        int64_t order_rel, double ksol, double length, double face_angle,
+                        double face_angle_feed_down, double fringe_integral,
+                        double factor_for_backtrack) {
+    double k0 = 0;
+    k0 += knorm[0];                                         /* k_order = 3 > -1 */
+    if (fabs(length) > 1e-10 && kl_order > -1) k0 += factor_knl_ksl * knl[0] / length;
+    if (fabs(length) > 1e-10 && order_rel > -1) k0 += factor_knl_ksl_rel * knl_rel[0] / length;
+    if (is_exit) {
+        FOR_ACTIVE(part0, part) { SET(ax, 0.); SET(ay, 0.); } END_FOR_ACTIVE
+    } else {
+        FOR_ACTIVE(part0, part) {
+            SET(ax, -0.5 * ksol * (GET(y) - 0.0));
+            SET(ay, 0.5 * ksol * (GET(x) - 0.0));
+        } END_FOR_ACTIVE
+    }
+    if (model == 0) {
+        double const corr = 2.0 * k0 * half_gap * fringe_integral;
+        double r21 = k0 * M_TAN(face_angle);
+        double const e1_v = face_angle + face_angle_feed_down;
+        double const sin_e1_v = M_SIN(e1_v);
+        double const temp = corr / M_COS(e1_v) * (1.0 + sin_e1_v * sin_e1_v);
+        double r43 = -k0 * M_TAN(e1_v - temp);
+        r21 = r21 * factor_for_backtrack;
+        r43 = r43 * factor_for_backtrack;
+        FOR_ACTIVE(part0, part) {
+            double const x = GET(x);
+            double const y = GET(y);
+            double const chi = GET(chi);
+            ADD(px, chi * r21 * x);
+            ADD(py, chi * r43 * y);
+        } END_FOR_ACTIVE
+    }
+}
+
+/* --------------------------------------------------------------------------
+ * Magnet elements: per-class argument mapping (multipole.h:16-69,
+ * quadrupole.h:15-75, bend.h:15-76, sextupole.h / octupole.h:15-75) then
+ * track_magnet_particles (track_magnet.h:289-648, no rbend, no radiation). */
+static int magnet_track(LP *part0, int type, const double *p) {
+    const int64_t order = (int64_t)p[XTM_M_ORDER];
+    const int64_t nrel = (int64_t)p[XTM_M_NREL];
+    const double *knl = p + XTM_M_ARRAYS;
+    const double *ksl = knl + (order + 1);
+    const double *knl_rel = ksl + (order + 1);
+    const double *ksl_rel = knl_rel + nrel;
+    double length = p[XTM_M_LENGTH];
+    double h = 0, hxl = 0, k0 = 0, k1 = 0, k2 = 0, k3 = 0, k0s = 0, k1s = 0, k2s = 0, k3s = 0;
+    double rel_ref_strength;
+    int64_t model = (int64_t)p[XTM_M_MODEL];
+    int64_t default_model, default_integrator;
+    int64_t edge_entry_active = 0, edge_exit_active = 0, edge_entry_model = 0, edge_exit_model = 0;
+    double edge_entry_angle = 0, edge_exit_angle = 0, edge_entry_fdown = 0, edge_exit_fdown = 0;
+    double edge_entry_fint = 0, edge_exit_fint = 0, edge_entry_hgap = 0, edge_exit_hgap = 0;
+    const int skew = p[XTM_M_MAIN_IS_SKEW] != 0;
+    switch (type) {
+    case XTM_MULTIPOLE: {
+        int64_t mo = (int64_t)p[XTM_M_MAIN_ORDER];
+        rel_ref_strength = skew ? ksl[mo] : knl[mo];
+        if (p[XTM_M_ISTHICK] <= 0) model = -1;
+        default_model = 6; default_integrator = 3;
+        hxl = p[XTM_M_HXL];
+        break; }
+    case XTM_QUADRUPOLE:
+        rel_ref_strength = length * (skew ? p[XTM_M_K1S] : p[XTM_M_K1]);
+        default_model = 4; default_integrator = 3;
+        k1 = p[XTM_M_K1]; k1s = p[XTM_M_K1S];
+        edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+        edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+        edge_entry_model = 1; edge_exit_model = 1;
+        break;
+    case XTM_BEND:
+        rel_ref_strength = p[XTM_M_K0] * length;
+        default_model = 3; default_integrator = 2;
+        h = p[XTM_M_H]; k0 = p[XTM_M_K0]; k1 = p[XTM_M_K1]; k2 = p[XTM_M_K2];
+        edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+        edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+        edge_entry_model = (int64_t)p[XTM_M_EDGE_ENTRY_MODEL];
+        edge_exit_model = (int64_t)p[XTM_M_EDGE_EXIT_MODEL];
+        edge_entry_angle = p[XTM_M_EDGE_ENTRY_ANGLE]; edge_exit_angle = p[XTM_M_EDGE_EXIT_ANGLE];
+        edge_entry_fdown = p[XTM_M_EDGE_ENTRY_ANGLE_FDOWN]; edge_exit_fdown = p[XTM_M_EDGE_EXIT_ANGLE_FDOWN];
+        edge_entry_fint = p[XTM_M_EDGE_ENTRY_FINT]; edge_exit_fint = p[XTM_M_EDGE_EXIT_FINT];
+        edge_entry_hgap = p[XTM_M_EDGE_ENTRY_HGAP]; edge_exit_hgap = p[XTM_M_EDGE_EXIT_HGAP];
+        break;
+    case XTM_SEXTUPOLE:
+        rel_ref_strength = length * (skew ? p[XTM_M_K2S] : p[XTM_M_K2]);
+        default_model = 6; default_integrator = 3;
+        k2 = p[XTM_M_K2]; k2s = p[XTM_M_K2S];
+        edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+        edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+        edge_entry_model = 1; edge_exit_model = 1;
+        break;
+    case XTM_OCTUPOLE:
+        rel_ref_strength = length * (skew ? p[XTM_M_K3S] : p[XTM_M_K3]);
+        default_model = 6; default_integrator = 3;
+        k3 = p[XTM_M_K3]; k3s = p[XTM_M_K3S];
+        edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+        edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+        edge_entry_model = 1; edge_exit_model = 1;
+        break;
+    default:
+        return XTM_ERR_ARG;
+    }
+    const double weight = 1.;
+    double const inv_factorial_order = p[XTM_M_INV_FACTORIAL_ORDER];
+    const int64_t order_rel = nrel - 1;
+    const double inv_factorial_order_rel = one_over_factorial(nrel - 1);
+    int64_t num_multipole_kicks = (int64_t)p[XTM_M_NUM_MULTIPOLE_KICKS];
+    int64_t integrator = (int64_t)p[XTM_M_INTEGRATOR];
+    if (p[XTM_M_RADIATION_FLAG] != 0) return XTM_ERR_UNSUPPORTED;
+
+    double factor_knl_ksl = 1.0;
+    double core_length, factor_knl_ksl_body, factor_knl_ksl_edge, factor_backtrack_edge;
+    if (check_flag(part0, XTM_FLAG_BACKTRACK)) {
+        core_length = -length * weight;
+        factor_knl_ksl_body = -factor_knl_ksl * weight;
+        factor_knl_ksl_edge = factor_knl_ksl;
+        factor_backtrack_edge = -1.;
+        hxl = -hxl;
+        int64_t ti; double td;
+        ti = edge_entry_active; edge_entry_active = edge_exit_active; edge_exit_active = ti;
+        ti = edge_entry_model; edge_entry_model = edge_exit_model; edge_exit_model = ti;
+        td = edge_entry_angle; edge_entry_angle = edge_exit_angle; edge_exit_angle = td;
+        td = edge_entry_fdown; edge_entry_fdown = edge_exit_fdown; edge_exit_fdown = td;
+        td = edge_entry_fint; edge_entry_fint = edge_exit_fint; edge_exit_fint = td;
+        td = edge_entry_hgap; edge_entry_hgap = edge_exit_hgap; edge_exit_hgap = td;
+    } else {
+        core_length = length * weight;
+        factor_knl_ksl_body = factor_knl_ksl * weight;
+        factor_knl_ksl_edge = factor_knl_ksl;
+        factor_backtrack_edge = 1.;
+    }
+    double knorm[4] = {k0, k1, k2, k3};
+    if (edge_entry_active) {
+        if (!(edge_entry_model == 0 || edge_entry_model == -1 || edge_entry_model == 3))
+            return XTM_ERR_UNSUPPORTED;
+        magnet_edge(part0, edge_entry_model, 0, edge_entry_hgap, knorm, factor_knl_ksl_edge,
+                    order, knl, knl_rel, factor_knl_ksl_edge * rel_ref_strength, order_rel,
+                    0.0, length, edge_entry_angle, edge_entry_fdown, edge_entry_fint,
+                    factor_backtrack_edge);
+    }
+    /* body */
+    if (integrator == 0) integrator = default_integrator;
+    if (model == 0) model = default_model;
+    if (model == -1) { integrator = 3; num_multipole_kicks = 1; }
+    if (num_multipole_kicks == 0) {
+        if (!kick_is_inactive(order, knl, ksl, k0, k1, k2, k3, k0s, k1s, k2s, k3s, h)) {
+            if (fabs(h) < 1e-8) {
+                num_multipole_kicks = 1;
+            } else {
+                double b_circum = 2 * 3.14159 / fabs(h);
+                num_multipole_kicks = (int64_t)(fabs(core_length) / b_circum / 0.5e-3);
+                if (num_multipole_kicks < 1) num_multipole_kicks = 1;
+            }
+        }
+    }
+    body_args a;
+    memset(&a, 0, sizeof(a));
+    int st = configure_tracking_model(model, k0, k1, h, &a);
+    if (st) return st;
+    a.length = core_length;
+    a.order = order; a.inv_factorial_order = inv_factorial_order; a.knl = knl; a.ksl = ksl;
+    a.order_rel = order_rel; a.inv_factorial_order_rel = inv_factorial_order_rel;
+    a.knl_rel = knl_rel; a.ksl_rel = ksl_rel;
+    a.rel_ref_strength = rel_ref_strength;
+    a.factor_knl_ksl = factor_knl_ksl_body;
+    a.num_multipole_kicks = num_multipole_kicks;
+    a.integrator = integrator;
+    a.hxl = hxl;
+    a.k2 = k2; a.k3 = k3; a.k0s = k0s; a.k1s = k1s; a.k2s = k2s; a.k3s = k3s;
+    FOR_ACTIVE(part0, part) {
+        magnet_body(part, &a);
+    } END_FOR_ACTIVE
+    if (edge_exit_active) {
+        if (!(edge_exit_model == 0 || edge_exit_model == -1 || edge_exit_model == 3))
+            return XTM_ERR_UNSUPPORTED;
+        magnet_edge(part0, edge_exit_model, 1, edge_exit_hgap, knorm, factor_knl_ksl_edge,
+                    order, knl, knl_rel, factor_knl_ksl_edge * rel_ref_strength, order_rel,
+                    0.0, length, edge_exit_angle, edge_exit_fdown, edge_exit_fint,
+                    factor_backtrack_edge);
+    }
+    return XTM_OK;
+}
+
+/* --------------------------------------------------------------------------
+ * Cavity: cavity.h:12-48 -> track_rf.h:317-453, kick :18-167 (order -1,
+ * no transverse voltage), energy update with pz_only = 1. */
+static void rf_kick(LP *part, double voltage, double frequency, double harmonic, double lag,
+                    double phase, int64_t absolute_time, int kill_energy_kick) {
+    double phase0 = 0;
+    double const beta0 = GET(beta0);
+    if (harmonic != 0) {
+        double const line_length = part->line_length;
+        double const t_rev0 = line_length / (beta0 * C_LIGHT);
+        frequency += (harmonic / t_rev0);
+    }
+    if (absolute_time == 1) {
+        double const t_sim = part->P->t_sim;
+        int64_t const at_turn = GET(at_turn);
+        phase0 += 2 * PI * at_turn * frequency * t_sim;
+    }
+    double const zeta = GET(zeta);
+    double const q = fabs(part->P->q0) * GET(charge_ratio);
+    double const tau = zeta / beta0;
+    double const energy_kick = q * voltage
+        * M_SIN(phase0 + DEG2RAD * lag + phase - (2.0 * PI) / C_LIGHT * frequency * tau);
+    double rfmultipole_energy_kick = 0;
+    if (!kill_energy_kick) add_to_energy(part, energy_kick + rfmultipole_energy_kick, 1);
+}
+
+static int cavity_track(LP *part0, const double *p) {
+    double length = p[XTM_C_LENGTH];
+    double voltage = p[XTM_C_VOLTAGE];
+    const double frequency = p[XTM_C_FREQUENCY];
+    const double harmonic = p[XTM_C_HARMONIC];
+    const double lag = p[XTM_C_LAG];
+    const double phase = p[XTM_C_PHASE];
+    const int64_t absolute_time = (int64_t)p[XTM_C_ABSOLUTE_TIME];
+    int64_t num_kicks = (int64_t)p[XTM_C_NUM_KICKS];
+    int64_t model = (int64_t)p[XTM_C_MODEL];
+    int64_t integrator = (int64_t)p[XTM_C_INTEGRATOR];
+    const double weight = 1.;
+    int kill_energy_kick = check_flag(part0, XTM_FLAG_KILL_CAVITY_KICK);
+    double body_length;
+    if (check_flag(part0, XTM_FLAG_BACKTRACK)) {
+        body_length = -length;
+        voltage = -voltage;
+    } else {
+        body_length = length;
+    }
+    if (integrator == 0) integrator = 3;
+    if (model == 0) model = 6;
+    if (model == -1) { integrator = 3; num_kicks = 1; }
+    if (num_kicks == 0) num_kicks = 1;
+    body_args cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    int st = configure_tracking_model(model, 0, 0, 0, &cfg);
+    if (st) return st;
+    const int64_t drift_model = cfg.drift_model;
+    const double blen = body_length * weight;
+    const double vw = voltage * weight;
+#define RFDRIFT(dl) magnet_drift(part, (dl), 0., 0., 0., drift_model)
+#define RFKICK(w) rf_kick(part, vw * (w), frequency, harmonic, lag, phase, absolute_time, kill_energy_kick)
+    FOR_ACTIVE(part0, part) {
+        if (integrator == 1) {
+            const double kick_weight = 1. / num_kicks;
+            double edge_drift_weight = 0.5;
+            double inside_drift_weight = 0;
+            if (num_kicks > 1) {
+                edge_drift_weight = 1. / (2 * (1 + num_kicks));
+                inside_drift_weight = (((double)num_kicks) / ((double)(num_kicks * num_kicks) - 1));
+            }
+            RFDRIFT(edge_drift_weight * blen);
+            for (int i_kick = 0; i_kick < num_kicks - 1; i_kick++) {
+                RFKICK(kick_weight);
+                RFDRIFT(inside_drift_weight * blen);
+            }
+            RFKICK(kick_weight);
+            RFDRIFT(edge_drift_weight * blen);
+        } else if (integrator == 3) {
+            const double kick_weight = 1. / num_kicks;
+            const double drift_weight = kick_weight;
+            for (int i_kick = 0; i_kick < num_kicks; i_kick++) {
+                RFDRIFT(0.5 * drift_weight * blen);
+                RFKICK(kick_weight);
+                RFDRIFT(0.5 * drift_weight * blen);
+            }
+        } else if (integrator == 2) {
+            const int64_t num_slices = (num_kicks / 7 + (num_kicks % 7 != 0));
+            const double slice_length = blen / (num_slices);
+            const double kick_weight = 1. / num_slices;
+            const double d_y[] = {3.922568052387799819591407413100e-01,
+                                  5.100434119184584780271052295575e-01,
+                                  -4.710533854097565531482416645304e-01,
+                                  6.875316825251809316199569366290e-02};
+            const double k_y[] = {7.845136104775599639182814826199e-01,
+                                  2.355732133593569921359289764951e-01,
+                                  -1.177679984178870098432412305556e+00,
+                                  1.315186320683906284756403692882e+00};
+            for (int ii = 0; ii < num_slices; ii++) {
+                RFDRIFT(slice_length * d_y[0]); RFKICK(kick_weight * k_y[0]);
+                RFDRIFT(slice_length * d_y[1]); RFKICK(kick_weight * k_y[1]);
+                RFDRIFT(slice_length * d_y[2]); RFKICK(kick_weight * k_y[2]);
+                RFDRIFT(slice_length * d_y[3]); RFKICK(kick_weight * k_y[3]);
+                RFDRIFT(slice_length * d_y[3]); RFKICK(kick_weight * k_y[2]);
+                RFDRIFT(slice_length * d_y[2]); RFKICK(kick_weight * k_y[1]);
+                RFDRIFT(slice_length * d_y[1]); RFKICK(kick_weight * k_y[0]);
+                RFDRIFT(slice_length * d_y[0]);
+            }
+        }
+    } END_FOR_ACTIVE
+#undef RFDRIFT
+#undef RFKICK
+    return XTM_OK;
+}
+
+/* --------------------------------------------------------------------------
+ * Apertures (limitrect.h:10-38, limitellipse.h:13-38) */
+static void limitrect_track(LP *part0, const double *p) {
+    if (check_flag(part0, XTM_FLAG_IGNORE_LOCAL_APERTURE)) return;
+    double const min_x = p[XTM_LR_MIN_X], max_x = p[XTM_LR_MAX_X];
+    double const min_y = p[XTM_LR_MIN_Y], max_y = p[XTM_LR_MAX_Y];
+    FOR_ACTIVE(part0, part) {
+        double const x = GET(x);
+        double const y = GET(y);
+        int64_t const is_alive = (int64_t)((x >= min_x) && (x <= max_x) && (y >= min_y) && (y <= max_y));
+        if (!is_alive) SET(state, 0);
+    } END_FOR_ACTIVE
+}
+
+static void limitellipse_track(LP *part0, const double *p) {
+    if (check_flag(part0, XTM_FLAG_IGNORE_LOCAL_APERTURE)) return;
+    double const a_squ = p[XTM_LE_A_SQU], b_squ = p[XTM_LE_B_SQU], a_b_squ = p[XTM_LE_A_B_SQU];
+    FOR_ACTIVE(part0, part) {
+        double const x = GET(x);
+        double const y = GET(y);
+        double const temp = x * x * b_squ + y * y * a_squ;
+        int64_t const is_alive = (int64_t)(temp <= a_b_squ);
+        if (!is_alive) SET(state, 0);
+    } END_FOR_ACTIVE
+}
+
+/* drift.h:14-40 */
+static void drift_track(LP *part0, const double *p) {
+    double length = p[XTM_D_LENGTH];
+    if (check_flag(part0, XTM_FLAG_BACKTRACK)) length = -length;
+    int64_t model = (int64_t)p[XTM_D_MODEL];
+    if (model == 0) model = 1;
+    if (model == 1) {
+        FOR_ACTIVE(part0, part) { drift_expanded(part, length); } END_FOR_ACTIVE
+    } else if (model == 2) {
+        FOR_ACTIVE(part0, part) { drift_exact(part, length); } END_FOR_ACTIVE
+    }
+}
+
+/* --------------------------------------------------------------------------
+ * monitors/particles_monitor.h:13-77 */
+static void copy_record(const xtm_particles *src, int64_t i, const xtm_particles *dst, int64_t j) {
+#define CP(f) dst->f[j] = src->f[i];
+    CP(p0c) CP(gamma0) CP(beta0) CP(s) CP(zeta) CP(x) CP(y) CP(px) CP(py) CP(ptau)
+    CP(delta) CP(rpp) CP(rvv) CP(chi) CP(charge_ratio) CP(weight) CP(ax) CP(ay)
+    CP(spin_x) CP(spin_y) CP(spin_z) CP(anomalous_magnetic_moment) CP(pdg_id)
+    CP(particle_id) CP(at_element) CP(at_turn) CP(state) CP(parent_particle_id)
+    CP(rng_s1) CP(rng_s2) CP(rng_s3) CP(rng_s4)
+#undef CP
+}
+
+static void monitor_track(const xtm_monitor *m, LP *part0) {
+    int64_t const n_turns_record = m->stop_at_turn - m->start_at_turn;
+    FOR_ACTIVE(part0, part) {
+        int64_t at_turn;
+        int skip = 0;
+        if (m->ebe_mode) {
+            at_turn = GET(at_element);
+        } else {
+            if (check_flag(part, XTM_FLAG_BACKTRACK)) { skip = 1; at_turn = 0; }
+            else at_turn = GET(at_turn);
+        }
+        if (!skip) {
+            if (m->n_repetitions == 1) {
+                if (at_turn >= m->start_at_turn && at_turn < m->stop_at_turn) {
+                    int64_t const pid = GET(particle_id);
+                    if (pid < m->part_id_end && pid >= m->part_id_start) {
+                        int64_t const store_at = n_turns_record * (pid - m->part_id_start)
+                                                 + at_turn - m->start_at_turn;
+                        copy_record(part->P, part->ipart, &m->data, store_at);
+                    }
+                }
+            } else if (m->n_repetitions > 1 && at_turn >= m->start_at_turn) {
+                int64_t const i_frame = (at_turn - m->start_at_turn) / m->repetition_period;
+                if (i_frame < m->n_repetitions
+                    && at_turn >= m->start_at_turn + i_frame * m->repetition_period
+                    && at_turn < m->stop_at_turn + i_frame * m->repetition_period) {
+                    int64_t const pid = GET(particle_id);
+                    if (pid < m->part_id_end && pid >= m->part_id_start) {
+                        int64_t const store_at =
+                            n_turns_record * (m->part_id_end - m->part_id_start) * i_frame
+                            + n_turns_record * (pid - m->part_id_start)
+                            + (at_turn - i_frame * m->repetition_period) - m->start_at_turn;
+                        copy_record(part->P, part->ipart, &m->data, store_at);
+                    }
+                }
+            }
+        }
+    } END_FOR_ACTIVE
+}
+
+/* --------------------------------------------------------------------------
+ * Element dispatch (the generated switch, tracker.py:646-695) */
+static int is_static_thick(int type) {
+    return type == XTM_DRIFT || type == XTM_QUADRUPOLE || type == XTM_BEND ||
+           type == XTM_SEXTUPOLE || type == XTM_OCTUPOLE || type == XTM_CAVITY;
+}
+
+static int track_element(LP *lp, int type, const double *p) {
+    int st = XTM_OK;
+    switch (type) {
+    case XTM_MARKER: break;
+    case XTM_DRIFT: drift_track(lp, p); break;
+    case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND:
+    case XTM_SEXTUPOLE: case XTM_OCTUPOLE: st = magnet_track(lp, type, p); break;
+    case XTM_CAVITY: st = cavity_track(lp, p); break;
+    case XTM_LIMITRECT: limitrect_track(lp, p); break;
+    case XTM_LIMITELLIPSE: limitellipse_track(lp, p); break;
+    default: return XTM_ERR_ARG;
+    }
+    if (st) return st;
+    if (is_static_thick(type)) global_aperture_check(lp);
+    return XTM_OK;
+}
+
+static void increment_at_element(LP *part0, int64_t inc) {
+    FOR_ACTIVE(part0, part) { part->P->at_element[part->ipart] += inc; } END_FOR_ACTIVE
+}
+
+static void increment_at_turn(LP *part0, int flag_reset_s) {
+    FOR_ACTIVE(part0, part) {
+        part->P->at_turn[part->ipart] += 1;
+        SET(at_element, 0);
+        if (flag_reset_s > 0) SET(s, 0.);
+    } END_FOR_ACTIVE
+}
+
+static void increment_at_turn_backtrack(LP *part0, int flag_reset_s, double line_length,
+                                        int64_t num_elements) {
+    FOR_ACTIVE(part0, part) {
+        part->P->at_turn[part->ipart] += -1;
+        SET(at_element, num_elements);
+        if (flag_reset_s > 0) SET(s, line_length);
+    } END_FOR_ACTIVE
+}
+
+/* --------------------------------------------------------------------------
+ * Public entry: the OpenMP `track_line` (tracker.py:546-748). */
+int32_t xto_track_line(const int32_t *types, const int64_t *offsets, const double *params,
+                       int64_t num_elements, double line_length, const xtm_particles *P,
+                       int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
+                       int32_t flag_end_turn_actions, int32_t flag_reset_s_at_end_turn,
+                       int32_t flag_monitor, const xtm_monitor *monitor, uint64_t track_flags,
+                       int32_t num_threads) {
+    if (ele_start < 0 || num_ele_track < 0 || ele_start + (int64_t)num_ele_track > num_elements)
+        return XTM_ERR_ARG;
+    if (flag_monitor && !monitor) return XTM_ERR_ARG;
+    if ((track_flags >> XTM_FLAG_SR_TAPER) & 1) return XTM_ERR_UNSUPPORTED;
+    if ((track_flags >> XTM_FLAG_BACKTRACK) & 1) return XTM_ERR_UNSUPPORTED;
+    const int64_t capacity = P->capacity;
+    if (num_threads < 1) num_threads = 1;
+#ifdef _OPENMP
+    omp_set_num_threads(num_threads);
+#else
+    num_threads = 1;
+#endif
+    /* reorganize before tracking (tracker.py:571-583) */
+    int64_t n_active, n_lost;
+    {
+        LP whole = {P, 0, capacity, track_flags, line_length};
+        if (capacity > 0) check_is_active(&whole);
+        count_reorganized(P, 0, capacity, &n_active, &n_lost);
+    }
+    const int64_t num_particles_to_track = n_active;
+    const int64_t chunk_size = (num_particles_to_track + num_threads - 1) / num_threads;
+    int status = XTM_OK;
+
+#pragma omp parallel for schedule(static, 1)
+    for (int chunk = 0; chunk < num_threads; chunk++) {
+        int64_t part_id = chunk * chunk_size;
+        int64_t end_id = (chunk + 1) * chunk_size;
+        if (end_id > num_particles_to_track) end_id = num_particles_to_track;
+        if (part_id >= capacity) continue;
+        LP lpart = {P, part_id, end_id, track_flags, line_length};
+        int64_t isactive = (part_id < end_id) ? check_is_active(&lpart) : 0;
+        for (int64_t iturn = 0; iturn < num_turns; iturn++) {
+            if (!isactive) break;
+            int64_t const ele_stop = ele_start + num_ele_track;
+            if (flag_monitor == 1) monitor_track(monitor, &lpart);
+            for (int64_t elem_idx = ele_start; elem_idx >= ele_start && elem_idx < ele_stop;
+                 elem_idx++) {
+                if (flag_monitor == 2) monitor_track(monitor, &lpart);
+                int st = track_element(&lpart, types[elem_idx], params + offsets[elem_idx]);
+                if (st) {
+#pragma omp critical
+                    status = st;
+                    isactive = 0;
+                    break;
+                }
+                isactive = check_is_active(&lpart);
+                if (!isactive) break;
+                increment_at_element(&lpart, 1);
+            }
+            if (flag_monitor == 2) monitor_track(monitor, &lpart);
+            if (flag_end_turn_actions > 0) {
+                if (isactive) increment_at_turn(&lpart, flag_reset_s_at_end_turn);
+            }
+        }
+    }
+    /* reorganize after tracking (tracker.py:741-748) */
+    {
+        LP whole = {P, 0, capacity, track_flags, line_length};
+        if (capacity > 0) check_is_active(&whole);
+    }
+    return status;
+}
+
+/* Element-wise evaluation of the elementary functions this build uses. */
+int32_t xto_math_eval(int32_t fn, const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; i++) {
+        double v = x[i];
+        switch (fn) {
+        case 0: y[i] = M_SIN(v); break;
+        case 1: y[i] = M_COS(v); break;
+        case 2: y[i] = M_SINH(v); break;
+        case 3: y[i] = M_COSH(v); break;
+        case 4: y[i] = M_EXP(v); break;
+        case 5: y[i] = M_TAN(v); break;
+        case 6: y[i] = sqrt(v); break;
+        case 7: y[i] = v / x[i + n]; break;
+        default: return XTM_ERR_ARG;
+        }
+    }
+    return XTM_OK;
+}
+
+int32_t xto_uses_libm(void) {
+#ifdef XTO_USE_LIBM
+    return 1;
+#else
+    return 0;
+#endif
+}

@@ -1,0 +1,641 @@
+// xt_builder.cpp -- raw element table -> device op programs (host side).
+//
+// For each element this file performs, once on the host, the per-element
+// steps the reference executes at every element call on every particle:
+// the per-class argument mapping (elements_src/{multipole,quadrupole,bend,
+// sextupole,octupole,cavity}.h), model/integrator defaults and the automatic
+// kick count (track_magnet.h:519-550), the model configuration
+// (track_magnet_configure.h:7-195), the integrator step sequence
+// (track_magnet.h:191-274, track_rf.h:198-310), the linear edge coefficients
+// (track_dipole_edge_linear.h:12-27) and the element-uniform factors of the
+// drift maps and kicks.  Every hoisted quantity is evaluated with the same
+// operation order and the same elementary functions (xt_math.h) as the
+// unhoisted expression, so the kernel reproduces the literal computation.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xtrack_amd.h"
+#include "xt_math.h"
+#include "xt_program.h"
+#include "xt_builder.h"
+
+namespace {
+
+double ibits(int64_t v) { double d; std::memcpy(&d, &v, 8); return d; }
+
+struct Prog {
+    std::vector<double> &p;
+    explicit Prog(std::vector<double> &v) : p(v) {}
+    void op(int o) { p.push_back(ibits(o)); }
+    void arg(double v) { p.push_back(v); }
+    size_t pos() const { return p.size(); }
+};
+
+// 1/n! table of headers/factorial.h (correctly rounded, n < 20), tgamma above
+double one_over_factorial(int64_t n) {
+    static const double t[20] = {
+        0x1p+0, 0x1p+0, 0x1p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+        0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+        0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+        0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
+        0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45,
+        0x1.952c77030ad4ap-49, 0x1.6827863b97d97p-53, 0x1.2f49b46814157p-57};
+    if (n < 0) return 0.;
+    if (n < 20) return t[n];
+    return 1. / std::tgamma((double)(n + 1));
+}
+
+struct MagnetArgs {
+    double length;
+    int64_t order; double inv_factorial_order; const double *knl, *ksl;
+    int64_t order_rel; double inv_factorial_order_rel; const double *knl_rel, *ksl_rel;
+    double rel_ref_strength;
+    int64_t num_multipole_kicks, model, default_model, integrator, default_integrator;
+    double h, hxl, k0, k1, k2, k3, k0s, k1s, k2s, k3s;
+    int64_t edge_entry_active, edge_exit_active, edge_entry_model, edge_exit_model;
+    double edge_entry_angle, edge_exit_angle, edge_entry_fdown, edge_exit_fdown;
+    double edge_entry_fint, edge_exit_fint, edge_entry_hgap, edge_exit_hgap;
+};
+
+struct ModelCfg {
+    int64_t drift_model;
+    double k0_drift, k1_drift, h_drift;
+    double k0_kick, k1_kick, h_kick, k0_h_correction, k1_h_correction;
+    int64_t kick_rot_frame;
+};
+
+// track_magnet_configure.h:7-195
+bool configure(int64_t model, double k0, double k1, double h, ModelCfg &c) {
+    if (model == 1) model = 3;
+    const bool h_is_zero = std::fabs(h) < 1e-8;
+    int64_t dm;
+    if (model == 3) dm = h_is_zero ? 1 : 7;
+    else if (model == 4) dm = 3;
+    else if (model == 5) dm = 1;
+    else if (model == 6) dm = 0;
+    else if (model == -1) dm = -1;
+    else if (model == 7) dm = h_is_zero ? 1 : 2;
+    else if (model == 8) dm = h_is_zero ? 1 : 8;
+    else return false;
+    c.drift_model = dm;
+    if (dm == -1 || dm == 0 || dm == 1) {
+        c.k0_drift = 0.0; c.k1_drift = 0.0; c.h_drift = 0.0;
+        c.k0_kick = k0; c.k1_kick = k1; c.h_kick = h;
+        c.k0_h_correction = k0; c.k1_h_correction = k1; c.kick_rot_frame = 1;
+    } else if (dm == 2) {
+        c.k0_drift = 0.0; c.k1_drift = 0.0; c.h_drift = h;
+        c.k0_kick = k0; c.k1_kick = k1; c.h_kick = h;
+        c.k0_h_correction = k0; c.k1_h_correction = k1; c.kick_rot_frame = 0;
+    } else if (dm == 3) {
+        c.k0_drift = k0; c.k1_drift = k1; c.h_drift = h;
+        c.k0_kick = 0.0; c.k1_kick = 0.0; c.h_kick = h;
+        c.k0_h_correction = 0.; c.k1_h_correction = k1; c.kick_rot_frame = 0;
+    } else {  // 7, 8
+        c.k0_drift = k0; c.k1_drift = 0.0; c.h_drift = h;
+        c.k0_kick = 0.0; c.k1_kick = k1; c.h_kick = h;
+        c.k0_h_correction = k0; c.k1_h_correction = k1; c.kick_rot_frame = 0;
+    }
+    return true;
+}
+
+// constants of one polar drift of length s (track_magnet_drift.h:62-70)
+void polar_consts(double h, double s, double *ca, double *sa, double *t2) {
+    double c, sn, s2, c2;
+    xt_sincos(h * s, &sn, &c);
+    xt_sincos(0.5 * h * s, &s2, &c2);
+    *ca = c;
+    *sa = sn;
+    *t2 = 2 * s2 * s2;
+}
+
+struct Builder {
+    std::vector<double> &prog;
+    std::string &err;
+    Builder(std::vector<double> &p, std::string &e) : prog(p), err(e) {}
+
+    // MAGNET_DRIFT / RF_DRIFT (track_magnet_drift.h:468-555)
+    bool drift(double dl, const ModelCfg &c) {
+        if (c.drift_model == -1) return true;
+        if (dl == 0.0) return true;
+        Prog P(prog);
+        const double h = c.h_drift;
+        switch (c.drift_model) {
+        case 0: P.op(OP_DRIFT_EXP); P.arg(dl); return true;
+        case 1: P.op(OP_DRIFT_EXACT); P.arg(dl); return true;
+        case 2: {
+            double ca, sa, t2;
+            polar_consts(h, dl, &ca, &sa, &t2);
+            P.op(OP_POLAR); P.arg(dl); P.arg(1 / h); P.arg(ca); P.arg(sa); P.arg(t2);
+            return true; }
+        case 3:
+            P.op(OP_DIPQUAD); P.arg(dl); P.arg(c.k0_drift); P.arg(c.k1_drift); P.arg(h);
+            return true;
+        case 7: {
+            const double a = 0.6756035959798289 * dl;
+            const double b = -0.17560359597982889 * dl;
+            double ca, sa, t2, cb, sb, tb;
+            polar_consts(h, a, &ca, &sa, &t2);
+            polar_consts(h, b, &cb, &sb, &tb);
+            P.op(OP_YOSHIDA4_BEND);
+            P.arg(dl); P.arg(1 / h);
+            P.arg(1.3512071919596578 * c.k0_drift);
+            P.arg((-1.7024143839193155) * c.k0_drift);
+            P.arg(a); P.arg(ca); P.arg(sa); P.arg(t2);
+            P.arg(b); P.arg(cb); P.arg(sb); P.arg(tb);
+            return true; }
+        case 8: {
+            static const double d[4] = {3.922568052387799819591407413100e-01,
+                                        5.100434119184584780271052295575e-01,
+                                        -4.710533854097565531482416645304e-01,
+                                        6.875316825251809316199569366290e-02};
+            static const double k[4] = {7.845136104775599639182814826199e-01,
+                                        2.355732133593569921359289764951e-01,
+                                        -1.177679984178870098432412305556e+00,
+                                        1.315186320683906284756403692882e+00};
+            P.op(OP_YOSHIDA6_BEND);
+            P.arg(dl); P.arg(1 / h);
+            for (int i = 0; i < 4; i++) P.arg(k[i] * c.k0_drift);
+            for (int i = 0; i < 4; i++) {
+                double s = d[i] * dl, cc, ss, tt;
+                polar_consts(h, s, &cc, &ss, &tt);
+                P.arg(s); P.arg(cc); P.arg(ss); P.arg(tt);
+            }
+            return true; }
+        default:
+            err = "drift model " + std::to_string(c.drift_model) + " is not supported";
+            return false;
+        }
+    }
+
+    // kick descriptor (track_magnet_kick.h:24-144) appended at the current end
+    size_t kick_desc(const MagnetArgs &m, const ModelCfg &c, double core_length,
+                     double factor) {
+        size_t base = prog.size();
+        prog.resize(base + KD_ARR, 0.0);
+        double *d = nullptr;
+        auto at = [&](int i) -> double & { return prog[base + i]; };
+        at(KD_ORDER) = ibits(m.order);
+        at(KD_ORDER_REL) = ibits(m.order_rel);
+        at(KD_LENGTH) = core_length;
+        at(KD_FACTOR) = factor;
+        at(KD_FACTOR_REL) = factor * m.rel_ref_strength;
+        at(KD_ROT_FRAME) = ibits(c.kick_rot_frame);
+        at(KD_H) = c.h_kick;
+        at(KD_HXL) = m.hxl;
+        at(KD_K0HC) = c.k0_h_correction;
+        at(KD_K1HC) = c.k1_h_correction;
+        double htot = c.h_kick;
+        if (core_length != 0) htot += m.hxl / core_length;
+        at(KD_HTOT) = htot;
+        double k0l = 0;
+        if (m.order >= 0) k0l = m.knl[0] * factor;
+        if (m.order_rel >= 0) k0l += m.knl_rel[0] * factor * m.rel_ref_strength;
+        double k1l = 0;
+        if (m.order >= 1) k1l = m.knl[1] * factor;
+        if (m.order_rel >= 1) k1l += m.knl_rel[1] * factor * m.rel_ref_strength;
+        at(KD_K0L) = k0l;
+        at(KD_K1L) = k1l;
+        const double km[4] = {c.k0_kick, c.k1_kick, m.k2, m.k3};
+        const double ks[4] = {m.k0s, m.k1s, m.k2s, m.k3s};
+        for (int i = 0; i < 4; i++) {
+            at(KD_MAIN + i) = km[i] * core_length;
+            at(KD_MAINS + i) = ks[i] * core_length;
+        }
+        (void)d;
+        auto chain = [&](int64_t order, double inv_f) {
+            // f[order] = inv_f, f[k-1] = f[k]*k; stored f[order] .. f[0]
+            std::vector<double> f(order + 1);
+            f[order] = inv_f;
+            for (int64_t k = order; k > 0; k--) f[k - 1] = f[k] * k;
+            for (int64_t k = order; k >= 0; k--) prog.push_back(f[k]);
+        };
+        chain(m.order, m.inv_factorial_order);
+        for (int64_t i = 0; i <= m.order; i++) prog.push_back(m.knl[i]);
+        for (int64_t i = 0; i <= m.order; i++) prog.push_back(m.ksl[i]);
+        chain(m.order_rel, m.inv_factorial_order_rel);
+        for (int64_t i = 0; i <= m.order_rel; i++) prog.push_back(m.knl_rel[i]);
+        for (int64_t i = 0; i <= m.order_rel; i++) prog.push_back(m.ksl_rel[i]);
+        chain(3, 1. / (3 * 2));
+        return base;
+    }
+
+    // linear edge (track_magnet_edge.h:28-80 model 0; ax/ay for -1, 3)
+    bool edge(const MagnetArgs &m, bool is_exit, int64_t model, double angle,
+              double fdown, double fint, double hgap, double factor_edge,
+              double fbe) {
+        if (!(model == 0 || model == -1 || model == 3)) {
+            err = "edge model " + std::to_string(model) + " is not supported";
+            return false;
+        }
+        Prog P(prog);
+        const double ksol = 0.0;
+        if (is_exit) {
+            P.op(OP_EDGE_CLEAR_AXAY);
+        } else {
+            P.op(OP_EDGE_AXAY); P.arg(-0.5 * ksol); P.arg(0.5 * ksol);
+        }
+        if (model == 0) {
+            double k0 = 0;
+            k0 += m.k0;                                   // knorm[0], k_order = 3
+            if (std::fabs(m.length) > 1e-10 && m.order > -1)
+                k0 += factor_edge * m.knl[0] / m.length;
+            if (std::fabs(m.length) > 1e-10 && m.order_rel > -1)
+                k0 += (factor_edge * m.rel_ref_strength) * m.knl_rel[0] / m.length;
+            double const corr = 2.0 * k0 * hgap * fint;
+            double r21 = k0 * xt_tan(angle);
+            double const e1_v = angle + fdown;
+            double const sin_e1_v = xt_sin(e1_v);
+            double const temp = corr / xt_cos(e1_v) * (1.0 + sin_e1_v * sin_e1_v);
+            double r43 = -k0 * xt_tan(e1_v - temp);
+            r21 = r21 * fbe;
+            r43 = r43 * fbe;
+            P.op(OP_EDGE_LINEAR); P.arg(r21); P.arg(r43);
+        }
+        return true;
+    }
+
+    bool magnet(int type, const double *p) {
+        MagnetArgs m{};
+        m.order = (int64_t)p[XTM_M_ORDER];
+        const int64_t nrel = (int64_t)p[XTM_M_NREL];
+        if (m.order < 0 || nrel < 1) { err = "invalid multipole order"; return false; }
+        m.knl = p + XTM_M_ARRAYS;
+        m.ksl = m.knl + (m.order + 1);
+        m.knl_rel = m.ksl + (m.order + 1);
+        m.ksl_rel = m.knl_rel + nrel;
+        m.length = p[XTM_M_LENGTH];
+        m.inv_factorial_order = p[XTM_M_INV_FACTORIAL_ORDER];
+        m.order_rel = nrel - 1;
+        m.inv_factorial_order_rel = one_over_factorial(nrel - 1);
+        m.num_multipole_kicks = (int64_t)p[XTM_M_NUM_MULTIPOLE_KICKS];
+        m.model = (int64_t)p[XTM_M_MODEL];
+        m.integrator = (int64_t)p[XTM_M_INTEGRATOR];
+        const bool skew = p[XTM_M_MAIN_IS_SKEW] != 0;
+        if (p[XTM_M_RADIATION_FLAG] != 0) { err = "radiation is not supported"; return false; }
+        bool static_thick = true;
+        switch (type) {
+        case XTM_MULTIPOLE: {
+            int64_t mo = (int64_t)p[XTM_M_MAIN_ORDER];
+            if (mo < 0 || mo > m.order) { err = "main_order out of range"; return false; }
+            m.rel_ref_strength = skew ? m.ksl[mo] : m.knl[mo];
+            if (p[XTM_M_ISTHICK] <= 0) m.model = -1;
+            m.default_model = 6; m.default_integrator = 3;
+            m.hxl = p[XTM_M_HXL];
+            static_thick = false;
+            break; }
+        case XTM_QUADRUPOLE:
+            m.rel_ref_strength = m.length * (skew ? p[XTM_M_K1S] : p[XTM_M_K1]);
+            m.default_model = 4; m.default_integrator = 3;
+            m.k1 = p[XTM_M_K1]; m.k1s = p[XTM_M_K1S];
+            m.edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+            m.edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+            m.edge_entry_model = 1; m.edge_exit_model = 1;
+            break;
+        case XTM_BEND:
+            m.rel_ref_strength = p[XTM_M_K0] * m.length;
+            m.default_model = 3; m.default_integrator = 2;
+            m.h = p[XTM_M_H]; m.k0 = p[XTM_M_K0]; m.k1 = p[XTM_M_K1]; m.k2 = p[XTM_M_K2];
+            m.edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+            m.edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+            m.edge_entry_model = (int64_t)p[XTM_M_EDGE_ENTRY_MODEL];
+            m.edge_exit_model = (int64_t)p[XTM_M_EDGE_EXIT_MODEL];
+            m.edge_entry_angle = p[XTM_M_EDGE_ENTRY_ANGLE];
+            m.edge_exit_angle = p[XTM_M_EDGE_EXIT_ANGLE];
+            m.edge_entry_fdown = p[XTM_M_EDGE_ENTRY_ANGLE_FDOWN];
+            m.edge_exit_fdown = p[XTM_M_EDGE_EXIT_ANGLE_FDOWN];
+            m.edge_entry_fint = p[XTM_M_EDGE_ENTRY_FINT];
+            m.edge_exit_fint = p[XTM_M_EDGE_EXIT_FINT];
+            m.edge_entry_hgap = p[XTM_M_EDGE_ENTRY_HGAP];
+            m.edge_exit_hgap = p[XTM_M_EDGE_EXIT_HGAP];
+            break;
+        case XTM_SEXTUPOLE:
+            m.rel_ref_strength = m.length * (skew ? p[XTM_M_K2S] : p[XTM_M_K2]);
+            m.default_model = 6; m.default_integrator = 3;
+            m.k2 = p[XTM_M_K2]; m.k2s = p[XTM_M_K2S];
+            m.edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+            m.edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+            m.edge_entry_model = 1; m.edge_exit_model = 1;
+            break;
+        case XTM_OCTUPOLE:
+            m.rel_ref_strength = m.length * (skew ? p[XTM_M_K3S] : p[XTM_M_K3]);
+            m.default_model = 6; m.default_integrator = 3;
+            m.k3 = p[XTM_M_K3]; m.k3s = p[XTM_M_K3S];
+            m.edge_entry_active = (int64_t)p[XTM_M_EDGE_ENTRY_ACTIVE];
+            m.edge_exit_active = (int64_t)p[XTM_M_EDGE_EXIT_ACTIVE];
+            m.edge_entry_model = 1; m.edge_exit_model = 1;
+            break;
+        default:
+            err = "not a magnet";
+            return false;
+        }
+        // forward tracking (track_magnet.h:444-450), weight = 1
+        const double weight = 1.0;
+        const double factor_knl_ksl = 1.0;
+        const double core_length = m.length * weight;
+        const double factor_body = factor_knl_ksl * weight;
+        const double factor_edge = factor_knl_ksl;
+        const double fbe = 1.;
+
+        if (m.edge_entry_active &&
+            !edge(m, false, m.edge_entry_model, m.edge_entry_angle, m.edge_entry_fdown,
+                  m.edge_entry_fint, m.edge_entry_hgap, factor_edge, fbe))
+            return false;
+
+        int64_t integrator = m.integrator, model = m.model, nmk = m.num_multipole_kicks;
+        if (integrator == 0) integrator = m.default_integrator;
+        if (model == 0) model = m.default_model;
+        if (model == -1) { integrator = 3; nmk = 1; }
+        if (nmk == 0) {
+            bool inactive = (m.h == 0 && m.k0 == 0 && m.k1 == 0 && m.k2 == 0 && m.k3 == 0 &&
+                             m.k0s == 0 && m.k1s == 0 && m.k2s == 0 && m.k3s == 0);
+            for (int64_t i = m.order; inactive && i >= 0; i--)
+                if (m.knl[i] != 0 || m.ksl[i] != 0) inactive = false;
+            if (!inactive) {
+                if (std::fabs(m.h) < 1e-8) {
+                    nmk = 1;
+                } else {
+                    double b_circum = 2 * 3.14159 / std::fabs(m.h);
+                    nmk = (int64_t)(std::fabs(core_length) / b_circum / 0.5e-3);
+                    if (nmk < 1) nmk = 1;
+                }
+            }
+        }
+        ModelCfg c{};
+        if (!configure(model, m.k0, m.k1, m.h, c)) {
+            err = "magnet model " + std::to_string(model) + " is not supported";
+            return false;
+        }
+        // body (track_magnet.h:177-274)
+        const double length = core_length;
+        if (nmk == 0 && c.k0_kick == 0 && c.k1_kick == 0 && c.h_kick == 0) {
+            if (!drift(length, c)) return false;
+        } else {
+            // reserve the kick ops' descriptor offset: emitted after OP_END
+            std::vector<std::pair<size_t, double>> kicks;   // (position of desc arg, weight)
+            auto kick = [&](double w) {
+                Prog P(prog);
+                P.op(OP_KICK); P.arg(w);
+                kicks.push_back({prog.size(), w});
+                P.arg(0.0);  // patched below
+            };
+            if (integrator == 1) {
+                const double kick_weight = 1. / nmk;
+                double edge_drift_weight = 0.5;
+                double inside_drift_weight = 0;
+                if (nmk > 1) {
+                    edge_drift_weight = 1. / (2 * (1 + nmk));
+                    inside_drift_weight = (((double)nmk) / ((double)(nmk * nmk) - 1));
+                }
+                if (!drift(edge_drift_weight * length, c)) return false;
+                for (int64_t i = 0; i < nmk - 1; i++) {
+                    kick(kick_weight);
+                    if (!drift(inside_drift_weight * length, c)) return false;
+                }
+                kick(kick_weight);
+                if (!drift(edge_drift_weight * length, c)) return false;
+            } else if (integrator == 3) {
+                const double kick_weight = 1. / nmk;
+                const double drift_weight = kick_weight;
+                for (int64_t i = 0; i < nmk; i++) {
+                    if (!drift(0.5 * drift_weight * length, c)) return false;
+                    kick(kick_weight);
+                    if (!drift(0.5 * drift_weight * length, c)) return false;
+                }
+            } else if (integrator == 2) {
+                const int64_t num_slices = (nmk / 7 + (nmk % 7 != 0));
+                const double slice_length = length / (num_slices);
+                const double kick_weight = 1. / num_slices;
+                static const double dy[4] = {3.922568052387799819591407413100e-01,
+                                             5.100434119184584780271052295575e-01,
+                                             -4.710533854097565531482416645304e-01,
+                                             6.875316825251809316199569366290e-02};
+                static const double ky[4] = {7.845136104775599639182814826199e-01,
+                                             2.355732133593569921359289764951e-01,
+                                             -1.177679984178870098432412305556e+00,
+                                             1.315186320683906284756403692882e+00};
+                static const int di[8] = {0, 1, 2, 3, 3, 2, 1, 0};
+                static const int ki[7] = {0, 1, 2, 3, 2, 1, 0};
+                for (int64_t s = 0; s < num_slices; s++) {
+                    for (int j = 0; j < 7; j++) {
+                        if (!drift(slice_length * dy[di[j]], c)) return false;
+                        kick(kick_weight * ky[ki[j]]);
+                    }
+                    if (!drift(slice_length * dy[di[7]], c)) return false;
+                }
+            } else {
+                err = "integrator " + std::to_string(integrator) + " is not supported";
+                return false;
+            }
+            // the descriptor goes after the element's OP_END; the caller
+            // appends OP_END, so stash kicks for patching
+            pending_kicks = kicks;
+            pending_desc = true;
+            pending_m = m;
+            pending_c = c;
+            pending_core_length = core_length;
+            pending_factor = factor_body;
+        }
+        if (m.edge_exit_active &&
+            !edge(m, true, m.edge_exit_model, m.edge_exit_angle, m.edge_exit_fdown,
+                  m.edge_exit_fint, m.edge_exit_hgap, factor_edge, fbe))
+            return false;
+        if (static_thick) Prog(prog).op(OP_GLOBAL_APER);
+        return true;
+    }
+
+    bool cavity(const double *p) {
+        double length = p[XTM_C_LENGTH];
+        double voltage = p[XTM_C_VOLTAGE];
+        const double frequency = p[XTM_C_FREQUENCY];
+        const double harmonic = p[XTM_C_HARMONIC];
+        const double lag = p[XTM_C_LAG];
+        const double phase = p[XTM_C_PHASE];
+        const int64_t absolute_time = (int64_t)p[XTM_C_ABSOLUTE_TIME];
+        int64_t num_kicks = (int64_t)p[XTM_C_NUM_KICKS];
+        int64_t model = (int64_t)p[XTM_C_MODEL];
+        int64_t integrator = (int64_t)p[XTM_C_INTEGRATOR];
+        const double weight = 1.;
+        const double body_length = length;
+        if (integrator == 0) integrator = 3;
+        if (model == 0) model = 6;
+        if (model == -1) { integrator = 3; num_kicks = 1; }
+        if (num_kicks == 0) num_kicks = 1;
+        ModelCfg c{};
+        if (!configure(model, 0, 0, 0, c)) {
+            err = "cavity model " + std::to_string(model) + " is not supported";
+            return false;
+        }
+        const double blen = body_length * weight;
+        const double vw = voltage * weight;
+        auto rfkick = [&](double w) {
+            Prog P(prog);
+            P.op(OP_RF_KICK);
+            P.arg(vw * (w)); P.arg(frequency); P.arg(harmonic); P.arg(lag); P.arg(phase);
+            P.arg(ibits(absolute_time));
+        };
+        if (integrator == 1) {
+            const double kick_weight = 1. / num_kicks;
+            double edge_drift_weight = 0.5;
+            double inside_drift_weight = 0;
+            if (num_kicks > 1) {
+                edge_drift_weight = 1. / (2 * (1 + num_kicks));
+                inside_drift_weight = (((double)num_kicks) / ((double)(num_kicks * num_kicks) - 1));
+            }
+            if (!drift(edge_drift_weight * blen, c)) return false;
+            for (int64_t i = 0; i < num_kicks - 1; i++) {
+                rfkick(kick_weight);
+                if (!drift(inside_drift_weight * blen, c)) return false;
+            }
+            rfkick(kick_weight);
+            if (!drift(edge_drift_weight * blen, c)) return false;
+        } else if (integrator == 3) {
+            const double kick_weight = 1. / num_kicks;
+            const double drift_weight = kick_weight;
+            for (int64_t i = 0; i < num_kicks; i++) {
+                if (!drift(0.5 * drift_weight * blen, c)) return false;
+                rfkick(kick_weight);
+                if (!drift(0.5 * drift_weight * blen, c)) return false;
+            }
+        } else if (integrator == 2) {
+            const int64_t num_slices = (num_kicks / 7 + (num_kicks % 7 != 0));
+            const double slice_length = blen / (num_slices);
+            const double kick_weight = 1. / num_slices;
+            static const double dy[4] = {3.922568052387799819591407413100e-01,
+                                         5.100434119184584780271052295575e-01,
+                                         -4.710533854097565531482416645304e-01,
+                                         6.875316825251809316199569366290e-02};
+            static const double ky[4] = {7.845136104775599639182814826199e-01,
+                                         2.355732133593569921359289764951e-01,
+                                         -1.177679984178870098432412305556e+00,
+                                         1.315186320683906284756403692882e+00};
+            static const int di[8] = {0, 1, 2, 3, 3, 2, 1, 0};
+            static const int ki[7] = {0, 1, 2, 3, 2, 1, 0};
+            for (int64_t s = 0; s < num_slices; s++) {
+                for (int j = 0; j < 7; j++) {
+                    if (!drift(slice_length * dy[di[j]], c)) return false;
+                    rfkick(kick_weight * ky[ki[j]]);
+                }
+                if (!drift(slice_length * dy[di[7]], c)) return false;
+            }
+        } else {
+            err = "integrator " + std::to_string(integrator) + " is not supported";
+            return false;
+        }
+        Prog(prog).op(OP_GLOBAL_APER);
+        return true;
+    }
+
+    // element -> ops; returns start offset or -1
+    int64_t element(int type, const double *p) {
+        const int64_t start = (int64_t)prog.size();
+        Prog P(prog);
+        pending_desc = false;
+        pending_kicks.clear();
+        bool ok = true;
+        switch (type) {
+        case XTM_MARKER: break;
+        case XTM_DRIFT: {
+            int64_t model = (int64_t)p[XTM_D_MODEL];
+            if (model == 0) model = 1;
+            if (model == 1) { P.op(OP_DRIFT_EXP); P.arg(p[XTM_D_LENGTH]); }
+            else if (model == 2) { P.op(OP_DRIFT_EXACT); P.arg(p[XTM_D_LENGTH]); }
+            else { err = "drift model not supported"; ok = false; }
+            P.op(OP_GLOBAL_APER);
+            break; }
+        case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND:
+        case XTM_SEXTUPOLE: case XTM_OCTUPOLE:
+            ok = magnet(type, p);
+            break;
+        case XTM_CAVITY: ok = cavity(p); break;
+        case XTM_LIMITRECT:
+            P.op(OP_LIMIT_RECT);
+            for (int i = 0; i < 4; i++) P.arg(p[i]);
+            break;
+        case XTM_LIMITELLIPSE:
+            P.op(OP_LIMIT_ELLIPSE);
+            for (int i = 0; i < 3; i++) P.arg(p[i]);
+            break;
+        default:
+            err = "unknown element type " + std::to_string(type);
+            ok = false;
+        }
+        if (!ok) return -1;
+        P.op(OP_END);
+        if (pending_desc) {
+            size_t desc = kick_desc(pending_m, pending_c, pending_core_length, pending_factor);
+            for (auto &k : pending_kicks) prog[k.first] = ibits((int64_t)desc);
+        }
+        return start;
+    }
+
+    bool pending_desc = false;
+    std::vector<std::pair<size_t, double>> pending_kicks;
+    MagnetArgs pending_m{};
+    ModelCfg pending_c{};
+    double pending_core_length = 0, pending_factor = 0;
+};
+
+size_t raw_size(int type, const double *p) {
+    switch (type) {
+    case XTM_MARKER: return 0;
+    case XTM_DRIFT: return XTM_D_SIZE;
+    case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND:
+    case XTM_SEXTUPOLE: case XTM_OCTUPOLE: {
+        int64_t order = (int64_t)p[XTM_M_ORDER], nrel = (int64_t)p[XTM_M_NREL];
+        return XTM_M_ARRAYS + 2 * (order + 1) + 2 * nrel; }
+    case XTM_CAVITY: return XTM_C_SIZE;
+    case XTM_LIMITRECT: return XTM_LR_SIZE;
+    case XTM_LIMITELLIPSE: return XTM_LE_SIZE;
+    default: return 0;
+    }
+}
+
+}  // namespace
+
+int xt_build_program(const int32_t *types, const int64_t *offsets, const double *params,
+                     int64_t n, std::vector<int64_t> &entry, std::vector<double> &prog,
+                     std::string &err) {
+    entry.assign(n, 0);
+    prog.clear();
+    prog.reserve(1024);
+    Builder B(prog, err);
+    std::map<std::string, int64_t> seen;   // raw record bytes -> program offset
+    for (int64_t i = 0; i < n; i++) {
+        const int type = types[i];
+        if (type < 0 || type >= XTM_NUM_TYPES) {
+            err = "element " + std::to_string(i) + ": unknown type";
+            return XTM_ERR_ARG;
+        }
+        const double *p = params + offsets[i];
+        if (type >= XTM_MULTIPOLE && type <= XTM_OCTUPOLE) {
+            if (p[XTM_M_ORDER] < 0 || p[XTM_M_NREL] < 1) {
+                err = "element " + std::to_string(i) + ": invalid multipole arrays";
+                return XTM_ERR_ARG;
+            }
+        }
+        const size_t sz = raw_size(type, p);
+        std::string key(reinterpret_cast<const char *>(&type), sizeof(type));
+        key.append(reinterpret_cast<const char *>(p), sz * sizeof(double));
+        auto it = seen.find(key);
+        if (it != seen.end()) { entry[i] = it->second; continue; }
+        int64_t off = B.element(type, p);
+        if (off < 0) {
+            err = "element " + std::to_string(i) + ": " + err;
+            return XTM_ERR_UNSUPPORTED;
+        }
+        seen.emplace(std::move(key), off);
+        entry[i] = off;
+    }
+    if (prog.empty()) prog.push_back(ibits(OP_END));
+    return XTM_OK;
+}
+
+int64_t xt_count_records(const std::vector<int64_t> &entry) {
+    std::map<int64_t, int> m;
+    for (auto e : entry) m[e] = 1;
+    return (int64_t)m.size();
+}

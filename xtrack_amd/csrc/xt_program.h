@@ -1,0 +1,79 @@
+/* xt_program.h -- device representation of a line (shared by the host
+ * builder and the HIP kernel).
+ *
+ * A line is an array `entry[num_elements]` of offsets into one FP64 array
+ * `prog`.  At each offset starts the op list of that element: an op word
+ * (an integer stored in the bits of a double) followed by the op's
+ * constants, repeated until OP_END.  Op words and constants are read with
+ * wave-uniform addresses (every lane of a wave tracks the same element), so
+ * they are scalar loads and the op switch is a scalar branch.
+ *
+ * The constants are the element-uniform sub-expressions of the reference's
+ * element code (e.g. cos(h*s), 1/h in the polar drift of
+ * track_magnet_drift.h:45-87, the 1/n! chain of the multipole kick in
+ * track_magnet_kick.h:183-228), evaluated on the host with the same
+ * operations and the same elementary-function library as the expression
+ * would have on the device, so the per-particle arithmetic that remains in
+ * the kernel produces the same bits as the unhoisted expression.
+ */
+#ifndef XT_PROGRAM_H
+#define XT_PROGRAM_H
+
+#include <stdint.h>
+
+enum xt_op {
+    OP_END = 0,
+    OP_DRIFT_EXP = 1,      /* L                                               */
+    OP_DRIFT_EXACT = 2,    /* L                                               */
+    OP_POLAR = 3,          /* L rho ca sa t2      (t2 = 2*sa2*sa2)            */
+    OP_DIPQUAD = 4,        /* L k0 k1 h                                        */
+    OP_YOSHIDA4_BEND = 5,  /* L rho c1k0 c2k0 | a ca sa t2 | b ca sa t2       */
+    OP_YOSHIDA6_BEND = 6,  /* L rho kc0..kc3 | 4 x (s ca sa t2)               */
+    OP_KICK = 7,           /* w desc                                           */
+    OP_EDGE_AXAY = 8,      /* ca cb   (ax = ca*(y - 0), ay = cb*(x - 0))       */
+    OP_EDGE_CLEAR_AXAY = 9,/*                                                  */
+    OP_EDGE_LINEAR = 10,   /* r21 r43                                          */
+    OP_RF_KICK = 11,       /* V freq harmonic lag phase abs_time              */
+    OP_LIMIT_RECT = 12,    /* min_x max_x min_y max_y                          */
+    OP_LIMIT_ELLIPSE = 13, /* a_squ b_squ a_b_squ                              */
+    OP_GLOBAL_APER = 14,   /*                                                  */
+    OP_NUM = 15
+};
+
+/* number of constants following each op word */
+#define XT_OP_NARGS_INIT {0, 1, 1, 5, 4, 12, 22, 2, 2, 0, 2, 6, 4, 3, 0}
+
+/* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
+ * OP_KICK through an absolute offset into prog:
+ *   KD_ORDER, KD_ORDER_REL       integers (bits)
+ *   KD_LENGTH                    core length
+ *   KD_FACTOR, KD_FACTOR_REL     factor_knl_ksl, factor_knl_ksl*rel_ref_strength
+ *   KD_ROT_FRAME                 integer
+ *   KD_H, KD_HXL, KD_K0HC, KD_K1HC
+ *   KD_HTOT                      h + hxl/length (length != 0) else h
+ *   KD_K0L, KD_K1L               k0l_mult, k1l_mult
+ *   KD_MAIN  .. +3               knl_main = {k0,k1,k2,k3}_kick * length
+ *   KD_MAINS .. +3               ksl_main
+ *   KD_ARR: pass 1: inv-factorial chain f[order..0], knl[0..order], ksl[0..order]
+ *           pass 2: f_rel[order_rel..0], knl_rel, ksl_rel
+ *           pass 3: f_main[3..0]
+ * The chain f[k] is the value inv_factorial holds when coefficient k is
+ * used: f[order] = inv_factorial_order, f[k-1] = f[k] * k. */
+#define KD_ORDER 0
+#define KD_ORDER_REL 1
+#define KD_LENGTH 2
+#define KD_FACTOR 3
+#define KD_FACTOR_REL 4
+#define KD_ROT_FRAME 5
+#define KD_H 6
+#define KD_HXL 7
+#define KD_K0HC 8
+#define KD_K1HC 9
+#define KD_HTOT 10
+#define KD_K0L 11
+#define KD_K1L 12
+#define KD_MAIN 13
+#define KD_MAINS 17
+#define KD_ARR 21
+
+#endif /* XT_PROGRAM_H */

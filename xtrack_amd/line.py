@@ -1,0 +1,104 @@
+"""``Line``: an ordered list of named elements plus the tracking front door
+``Line.track`` (xtrack/line.py:1902-1993) and the default configuration of
+the reference (line.py:283-299): no synchrotron radiation, global |x|,|y|
+limit of 1 m after statically thick elements, s reset at the end of each
+turn."""
+from .elements import ELEMENT_CLASSES
+from .particles import Particles
+from .tracker import Tracker
+
+
+class Line:
+    def __init__(self, elements=(), element_names=None, particle_ref=None):
+        if isinstance(elements, dict):
+            element_names = list(elements.keys())
+            elements = list(elements.values())
+        elements = list(elements)
+        if element_names is None:
+            element_names = [f'e{ii}' for ii in range(len(elements))]
+        if len(element_names) != len(elements):
+            raise ValueError('elements and element_names differ in length')
+        self.elements = elements
+        self.element_names = list(element_names)
+        self.particle_ref = particle_ref
+        self.tracker = None
+        self.reset_s_at_end_turn = True
+        self.skip_end_turn_actions = False
+
+    def __len__(self):
+        return len(self.elements)
+
+    def __getitem__(self, key):
+        if isinstance(key, str):
+            return self.elements[self.element_names.index(key)]
+        return self.elements[key]
+
+    def get_length(self):
+        """Sum of lengths of thick elements."""
+        tot = 0.0
+        for ee in self.elements:
+            if ee.isthick:
+                tot += getattr(ee, 'length', 0.0)
+        return tot
+
+    def build_tracker(self, _context=None, backend=None, device=None):
+        """Build the device element table.  ``_context``/``device`` pick the
+        HIP device (default ``cuda:0``); ``backend`` substitutes another
+        implementation of the two-method backend protocol."""
+        dev = device or (_context if isinstance(_context, str) else None) or 'cuda:0'
+        self.tracker = Tracker(self, backend=backend, device=dev)
+        return self.tracker
+
+    def discard_tracker(self):
+        self.tracker = None
+
+    @property
+    def record_last_track(self):
+        return self.tracker.record_last_track if self.tracker else None
+
+    @property
+    def time_last_track(self):
+        return self.tracker.time_last_track if self.tracker else None
+
+    @property
+    def track_flags(self):
+        return self.tracker.track_flags
+
+    def track(self, particles, ele_start=0, ele_stop=None, num_elements=None,
+              num_turns=None, turn_by_turn_monitor=None, freeze_longitudinal=False,
+              time=False, with_progress=False, backtrack=False):
+        """Track ``particles`` in place (semantics of xtrack Line.track)."""
+        if self.tracker is None:
+            self.build_tracker()
+        return self.tracker._track(particles, ele_start=ele_start, ele_stop=ele_stop,
+                                   num_elements=num_elements, num_turns=num_turns,
+                                   turn_by_turn_monitor=turn_by_turn_monitor,
+                                   freeze_longitudinal=freeze_longitudinal,
+                                   backtrack=backtrack, time=time,
+                                   with_progress=with_progress)
+
+    def build_particles(self, **kwargs):
+        """Particles with the reference quantities of ``particle_ref`` and the
+        given physical coordinates."""
+        ref = self.particle_ref
+        if ref is None:
+            raise ValueError('line.particle_ref is not set')
+        kw = dict(q0=ref.q0, mass0=ref.mass0, p0c=float(ref._np('p0c')[0]))
+        kw.update(kwargs)
+        return Particles(**kw)
+
+    def to_dict(self):
+        return {'elements': {nn: ee.to_dict() for nn, ee in zip(self.element_names, self.elements)},
+                'element_names': list(self.element_names)}
+
+    @classmethod
+    def from_dict(cls, dct):
+        names = dct.get('element_names') or list(dct['elements'].keys())
+        elems = []
+        for nn in names:
+            ed = dict(dct['elements'][nn])
+            cls_name = ed.pop('__class__')
+            if cls_name not in ELEMENT_CLASSES:
+                raise NotImplementedError(f'element class {cls_name} is not supported')
+            elems.append(ELEMENT_CLASSES[cls_name](**ed))
+        return cls(elements=elems, element_names=names)
