@@ -180,7 +180,9 @@ int32_t xtm_device_count(void);
 /* Build the device element table of a line.  Replaces TrackerData's element
  * buffer and ElementRefData (xtrack/tracker_data.py:67-172, 237-255) and the
  * per-class parameter derivation done at kernel entry (e.g.
- * elements_src/quadrupole.h:15-75).  `device` is the HIP device ordinal.
+ * elements_src/quadrupole.h:15-75).  `device` is the HIP device ordinal;
+ * device = -1 builds a host-only handle that can be inspected
+ * (xtm_line_num_records, xtm_line_count_ops) but not tracked.
  * The arrays are read once; the caller keeps ownership. */
 int32_t xtm_line_create(const int32_t *types, const int64_t *offsets,
                         const double *params, int64_t num_elements,
@@ -206,6 +208,15 @@ int32_t xtm_track_line(xtm_line *line, const xtm_particles *particles,
                        int32_t flag_reset_s_at_end_turn, int32_t flag_monitor,
                        const xtm_monitor *monitor, uint64_t track_flags,
                        void *stream);
+
+/* Operation count of one turn of the kernel's element maps for one
+ * particle with 6D coordinates coords = (x, px, y, py, zeta, delta) and
+ * chi = 1, obtained by running the same element code on the host with an
+ * operation-counting scalar.  out[0..5]: add/sub, mul, div, sqrt,
+ * elementary-function calls, total arithmetic.  No counterpart in the
+ * reference; used for the algorithmic-FLOP roofline. */
+int32_t xtm_line_count_ops(const xtm_line *line, const double *coords, double p0c,
+                           double mass0, double q0, double *out);
 
 /* Wait for all work queued on `stream`. */
 int32_t xtm_synchronize(void *stream);

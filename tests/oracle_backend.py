@@ -76,3 +76,47 @@ class OracleBackend:
 
     def monitor_context(self):
         return None
+
+
+_host_kernel = None
+
+
+def host_kernel_lib():
+    """tests/hostkernel/_build/libkernel_host.so: the kernel's element code
+    compiled for the host (test infrastructure)."""
+    global _host_kernel
+    if _host_kernel is not None:
+        return _host_kernel
+    d = os.path.join(ROOT, 'tests', 'hostkernel')
+    subprocess.run(['make', '-C', d], check=True, capture_output=True)
+    lib = ctypes.CDLL(os.path.join(d, '_build', 'libkernel_host.so'))
+    c = ctypes
+    lib.xth_track_line.restype = c.c_int32
+    lib.xth_track_line.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int64, c.c_double,
+                                   c.POINTER(_native.XtmParticles), c.c_int32, c.c_int32,
+                                   c.c_int32, c.c_int32, c.c_int32, c.c_int32,
+                                   c.POINTER(_native.XtmMonitor), c.c_uint64]
+    _host_kernel = lib
+    return lib
+
+
+class HostKernelBackend(OracleBackend):
+    """Same protocol, running the kernel's own element code on the host."""
+
+    def __init__(self):
+        self.lib = host_kernel_lib()
+
+    def track_line(self, handle, particles, num_turns, ele_start, num_ele_track,
+                   flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor,
+                   monitor, track_flags):
+        types, offsets, params, line_length = handle
+        pst = _native.make_particles_struct(particles)
+        mst = _native.make_monitor_struct(monitor) if monitor is not None else None
+        st = self.lib.xth_track_line(
+            types.ctypes.data, offsets.ctypes.data, params.ctypes.data, len(types),
+            line_length, ctypes.byref(pst), int(num_turns), int(ele_start),
+            int(num_ele_track), int(flag_end_turn_actions), int(flag_reset_s_at_end_turn),
+            int(flag_monitor), ctypes.byref(mst) if mst is not None else None,
+            int(track_flags))
+        if st != 0:
+            raise RuntimeError(f'host kernel status {st}')

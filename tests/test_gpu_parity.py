@@ -45,3 +45,127 @@ def test_losses_on_every_element_kind():
     assert (sc['state'] <= 0).sum() > 100
     assert len(np.unique(sc['at_element'][sc['state'] <= 0])) > 3
     assert not bad, bad
+
+
+def test_ions_chi_not_one_literal_path():
+    """chi != 1 takes the literal kick on every lane."""
+    print('[ions]', flush=True)
+    rng = np.random.default_rng(11)
+    n = 1500
+    p = xt.Particles(p0c=7e9, x=rng.normal(0, 2e-3, n), px=rng.normal(0, 2e-4, n),
+                     y=rng.normal(0, 2e-3, n), py=rng.normal(0, 2e-4, n),
+                     zeta=rng.normal(0, 0.05, n), delta=rng.normal(0, 1e-3, n),
+                     charge_ratio=rng.choice([0.5, 1.0, 2.0], n),
+                     mass_ratio=rng.choice([1.0, 2.0, 4.0], n))
+    lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=2), p)
+    lg.track(pg, num_turns=10)
+    lc.track(pc, num_turns=10)
+    assert not compare_exact(pg, pc)
+
+
+def test_nonfinite_and_extreme_particles():
+    """NaN / inf / huge coordinates: identical results and loss records."""
+    print('[extreme]', flush=True)
+    rng = np.random.default_rng(12)
+    n = 600
+    x = rng.normal(0, 2e-3, n)
+    px = rng.normal(0, 2e-4, n)
+    x[:50] = np.nan
+    px[50:100] = np.inf
+    px[100:150] = rng.uniform(0.5, 3.0, 50)
+    x[150:200] = rng.uniform(-0.99, 0.99, 50)
+    p = xt.Particles(p0c=7e9, x=x, px=px, y=rng.normal(0, 1e-3, n), delta=rng.normal(0, 1e-3, n))
+    lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=2, aperture=0.5), p)
+    lg.track(pg, num_turns=3)
+    lc.track(pc, num_turns=3)
+    assert not compare_exact(pg, pc)
+    s = pc.sorted_by_id()
+    assert set(np.unique(s['state'])) >= {-1, 0}
+
+
+def test_global_aperture_after_static_thick_elements():
+    """xtrack tests/test_apertures.py:16-43 on the GPU."""
+    def mk():
+        return xt.Line(elements=[xt.Drift(length=2), xt.Quadrupole(length=2, k1=0),
+                                 xt.Multipole(length=2, isthick=True)])
+    line = mk()
+    line.build_tracker(device='cuda:0')
+    for start, expected in ((1, -1), (2, 1), (0, -1)):
+        p = xt.Particles(p0c=7e12, px=0.6)
+        p.move(_context='cuda:0')
+        line.track(p, ele_start=start, num_elements=1)
+        assert int(p.state.cpu()[0]) == expected
+
+
+def test_partial_tracking_matrix():
+    """tests/test_tracker.py:152-340: every (start, stop, turns) split."""
+    n_elem = 9
+    line = xt.Line(elements=[xt.Drift(length=1.) for _ in range(n_elem)])
+    line.build_tracker(device='cuda:0')
+    p0 = xt.Particles(x=[1e-3, -2e-3, 5e-3], y=[2e-3, -4e-3, 3e-3], zeta=1e-2, p0c=7e12)
+
+    def run(expected_turn, expected_el, **kw):
+        p = p0.copy(_context='cuda:0')
+        if 'ele_start' in kw:
+            p.at_element = kw['ele_start']
+            p.s = float(kw['ele_start'])
+        line.track(p, turn_by_turn_monitor=True, **kw)
+        h = p.to_numpy()
+        assert np.all(h['at_turn'] == expected_turn), kw
+        assert np.all(h['at_element'] == expected_el), kw
+        assert np.all(h['s'] == expected_el), kw
+        nmon = expected_turn if expected_el == 0 else expected_turn + 1
+        assert line.record_last_track.x.shape == (3, nmon), kw
+
+    for turns in (1, 2, 10):
+        run(turns, 0, num_turns=turns)
+        for start in range(n_elem):
+            run(turns, 0, num_turns=turns, ele_start=start)
+        for stop in range(1, n_elem):
+            run(turns - 1, stop, num_turns=turns, ele_stop=stop)
+        for start in range(n_elem):
+            for stop in range(start + 1, n_elem):
+                run(turns - 1, stop, num_turns=turns, ele_start=start, ele_stop=stop)
+            for stop in range(start + 1):
+                run(turns, stop, num_turns=turns, ele_start=start, ele_stop=stop)
+    for start in range(n_elem):
+        for shift in range(1, 3 * n_elem + 1):
+            t = (start + shift) // n_elem
+            run(t, start + shift - n_elem * t, ele_start=start, num_elements=shift)
+
+
+@pytest.mark.parametrize('ele_start,ele_stop,expected_x', [
+    (None, None, [0, 0.005, 0.010, 0.015, 0.020, 0.025]),
+    (None, 3, [0, 0.005, 0.010, 0.015, 0.020, 0.023]),
+    (2, None, [0, 0.003, 0.008, 0.013, 0.018, 0.023]),
+    (2, 3, [0, 0.003, 0.008, 0.013, 0.018, 0.021]),
+    (3, 2, [0, 0.002, 0.007, 0.012, 0.017, 0.022, 0.024]),
+])
+def test_tbt_monitor_series(ele_start, ele_stop, expected_x):
+    """Expected series of xtrack tests/test_tracker.py:577-602."""
+    line = xt.Line(elements=[xt.Drift(length=1)] * 5)
+    line.build_tracker(device='cuda:0')
+    p = xt.Particles(px=0.001)
+    p.move(_context='cuda:0')
+    line.track(p, num_turns=5, turn_by_turn_monitor=True, ele_start=ele_start,
+               ele_stop=ele_stop)
+    rec = line.record_last_track.x
+    assert rec.shape == (1, len(expected_x) - 1)
+    got = np.concatenate([rec[0], p.x.cpu().numpy()])
+    assert np.allclose(got, expected_x, atol=1e-16, rtol=0)
+
+
+def test_monitors_bit_exact():
+    """Turn-by-turn and element-by-element records, GPU vs oracle."""
+    p = random_particles(300, seed=5, scale=3e-3)
+    for mode in (True, 'ONE_TURN_EBE'):
+        lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=2), p)
+        nt = 6 if mode is True else 1
+        lg.track(pg, num_turns=nt, turn_by_turn_monitor=mode)
+        lc.track(pc, num_turns=nt, turn_by_turn_monitor=mode)
+        mg, mc = lg.record_last_track, lc.record_last_track
+        for ff in ('x', 'px', 'y', 'py', 'zeta', 'delta', 'at_turn', 'at_element', 'state',
+                   'particle_id'):
+            a, b = getattr(mg, ff), getattr(mc, ff)
+            assert a.shape == b.shape
+            assert np.array_equal(a, b, equal_nan=True), (mode, ff)

@@ -114,6 +114,9 @@ def load():
     lib.xtm_track_line.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32, c.c_int32,
                                    c.c_int32, c.c_int32, c.c_int32, c.c_int32,
                                    c.POINTER(XtmMonitor), c.c_uint64, c.c_void_p]
+    lib.xtm_line_count_ops.restype = c.c_int32
+    lib.xtm_line_count_ops.argtypes = [c.c_void_p, c.c_void_p, c.c_double, c.c_double,
+                                       c.c_double, c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

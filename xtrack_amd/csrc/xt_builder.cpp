@@ -176,7 +176,6 @@ struct Builder {
                      double factor) {
         size_t base = prog.size();
         prog.resize(base + KD_ARR, 0.0);
-        double *d = nullptr;
         auto at = [&](int i) -> double & { return prog[base + i]; };
         at(KD_ORDER) = ibits(m.order);
         at(KD_ORDER_REL) = ibits(m.order_rel);
@@ -205,7 +204,6 @@ struct Builder {
             at(KD_MAIN + i) = km[i] * core_length;
             at(KD_MAINS + i) = ks[i] * core_length;
         }
-        (void)d;
         auto chain = [&](int64_t order, double inv_f) {
             // f[order] = inv_f, f[k-1] = f[k]*k; stored f[order] .. f[0]
             std::vector<double> f(order + 1);
@@ -220,7 +218,44 @@ struct Builder {
         for (int64_t i = 0; i <= m.order_rel; i++) prog.push_back(m.knl_rel[i]);
         for (int64_t i = 0; i <= m.order_rel; i++) prog.push_back(m.ksl_rel[i]);
         chain(3, 1. / (3 * 2));
+        // fast blocks: a[k] = (knl[k]*factor)*f[k] for chi == 1
+        auto fast = [&](int slot, int64_t order, double inv_f, const double *knl,
+                        const double *ksl, double fac) {
+            std::vector<double> f(order + 1), a(order + 1), b(order + 1);
+            f[order] = inv_f;
+            for (int64_t k = order; k > 0; k--) f[k - 1] = f[k] * k;
+            int64_t top = -1;
+            for (int64_t k = 0; k <= order; k++) {
+                a[k] = (knl[k] * fac) * f[k];
+                b[k] = (ksl[k] * fac) * f[k];
+                if (a[k] != 0.0 || b[k] != 0.0) top = k;
+            }
+            prog[base + KD_FASTOFF + slot] = ibits((int64_t)prog.size());
+            prog.push_back(ibits(top));
+            for (int64_t k = 0; k <= top; k++) prog.push_back(a[k]);
+            for (int64_t k = 0; k <= top; k++) prog.push_back(b[k]);
+        };
+        fast(0, m.order, m.inv_factorial_order, m.knl, m.ksl, factor);
+        fast(1, m.order_rel, m.inv_factorial_order_rel, m.knl_rel, m.ksl_rel,
+             factor * m.rel_ref_strength);
+        const double kmL[4] = {km[0] * core_length, km[1] * core_length,
+                               km[2] * core_length, km[3] * core_length};
+        const double ksL[4] = {ks[0] * core_length, ks[1] * core_length,
+                               ks[2] * core_length, ks[3] * core_length};
+        fast(2, 3, 1. / (3 * 2), kmL, ksL, 1.0);
         return base;
+    }
+
+    // highest index with a non-zero fast coefficient of one pass (-1: none)
+    static int64_t pass_top(int64_t order, double inv_f, const double *knl, const double *ksl,
+                            double fac) {
+        std::vector<double> f(order + 1);
+        f[order] = inv_f;
+        for (int64_t k = order; k > 0; k--) f[k - 1] = f[k] * k;
+        int64_t top = -1;
+        for (int64_t k = 0; k <= order; k++)
+            if ((knl[k] * fac) * f[k] != 0.0 || (ksl[k] * fac) * f[k] != 0.0) top = k;
+        return top;
     }
 
     // linear edge (track_magnet_edge.h:28-80 model 0; ax/ay for -1, 3)
@@ -376,11 +411,42 @@ struct Builder {
         } else {
             // reserve the kick ops' descriptor offset: emitted after OP_END
             std::vector<std::pair<size_t, double>> kicks;   // (position of desc arg, weight)
+            // element-uniform parts of the kick (track_magnet_kick.h:24-144)
+            const double km[4] = {c.k0_kick * core_length, c.k1_kick * core_length,
+                                  m.k2 * core_length, m.k3 * core_length};
+            const double ks[4] = {m.k0s * core_length, m.k1s * core_length,
+                                  m.k2s * core_length, m.k3s * core_length};
+            const int64_t top1 = pass_top(m.order, m.inv_factorial_order, m.knl, m.ksl,
+                                          factor_body);
+            const int64_t top2 = pass_top(m.order_rel, m.inv_factorial_order_rel, m.knl_rel,
+                                          m.ksl_rel, factor_body * m.rel_ref_strength);
+            const int64_t top3 = pass_top(3, 1. / (3 * 2), km, ks, 1.0);
+            double htot = c.h_kick;
+            if (core_length != 0) htot += m.hxl / core_length;
+            double k0l = 0;
+            if (m.order >= 0) k0l = m.knl[0] * factor_body;
+            if (m.order_rel >= 0) k0l += m.knl_rel[0] * factor_body * m.rel_ref_strength;
+            double k1l = 0;
+            if (m.order >= 1) k1l = m.knl[1] * factor_body;
+            if (m.order_rel >= 1) k1l += m.knl_rel[1] * factor_body * m.rel_ref_strength;
+            const double k0c = c.k0_h_correction * core_length + k0l;
+            const double k1c = c.k1_h_correction * core_length + k1l;
             auto kick = [&](double w) {
+                const double hl = c.kick_rot_frame ? c.h_kick * core_length * w + m.hxl * w : 0.0;
+                const double g0 = ((-k0c) * w) * htot;
+                const double h1 = (htot * k1c) * w;
+                int64_t mask = 0;
+                if (top1 >= 0) mask |= 1;
+                if (top2 >= 0) mask |= 2;
+                if (top3 >= 0) mask |= 4;
+                if (c.kick_rot_frame && hl != 0.0) mask |= 8;
+                if (g0 != 0.0) mask |= 16;
+                if (h1 != 0.0) mask |= 32;
                 Prog P(prog);
                 P.op(OP_KICK); P.arg(w);
                 kicks.push_back({prog.size(), w});
-                P.arg(0.0);  // patched below
+                P.arg(0.0);  // descriptor offset, patched below
+                P.arg(ibits(mask)); P.arg(hl); P.arg(g0); P.arg(h1);
             };
             if (integrator == 1) {
                 const double kick_weight = 1. / nmk;

@@ -29,7 +29,7 @@ enum xt_op {
     OP_DIPQUAD = 4,        /* L k0 k1 h                                        */
     OP_YOSHIDA4_BEND = 5,  /* L rho c1k0 c2k0 | a ca sa t2 | b ca sa t2       */
     OP_YOSHIDA6_BEND = 6,  /* L rho kc0..kc3 | 4 x (s ca sa t2)               */
-    OP_KICK = 7,           /* w desc                                           */
+    OP_KICK = 7,           /* w desc mask hl g0 h1                            */
     OP_EDGE_AXAY = 8,      /* ca cb   (ax = ca*(y - 0), ay = cb*(x - 0))       */
     OP_EDGE_CLEAR_AXAY = 9,/*                                                  */
     OP_EDGE_LINEAR = 10,   /* r21 r43                                          */
@@ -41,7 +41,8 @@ enum xt_op {
 };
 
 /* number of constants following each op word */
-#define XT_OP_NARGS_INIT {0, 1, 1, 5, 4, 12, 22, 2, 2, 0, 2, 6, 4, 3, 0}
+#define XT_NARGS_KICK 6
+#define XT_OP_NARGS_INIT {0, 1, 1, 5, 4, 12, 22, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0}
 
 /* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
  * OP_KICK through an absolute offset into prog:
@@ -58,7 +59,17 @@ enum xt_op {
  *           pass 2: f_rel[order_rel..0], knl_rel, ksl_rel
  *           pass 3: f_main[3..0]
  * The chain f[k] is the value inv_factorial holds when coefficient k is
- * used: f[order] = inv_factorial_order, f[k-1] = f[k] * k. */
+ * used: f[order] = inv_factorial_order, f[k-1] = f[k] * k.
+ *
+ * Fast blocks (one per pass, offsets in KD_FASTOFF..+2): [top, a[0..top],
+ * b[0..top]] with a[k] = (knl[k]*factor)*f[k], b[k] = (ksl[k]*factor)*f[k]
+ * (the literal this_knl*inv_factorial for chi == 1) and top the highest
+ * index with a non-zero coefficient.
+ *
+ * OP_KICK args: w, desc offset, mask of present folded terms (KF_* in
+ * xt_ops.h), hl = h*length*w + hxl*w, g0 = ((-k0c)*w)*htot,
+ * h1 = (htot*k1c)*w with k0c = k0_h_correction*length + k0l_mult and
+ * k1c = k1_h_correction*length + k1l_mult. */
 #define KD_ORDER 0
 #define KD_ORDER_REL 1
 #define KD_LENGTH 2
@@ -74,6 +85,7 @@ enum xt_op {
 #define KD_K1L 12
 #define KD_MAIN 13
 #define KD_MAINS 17
-#define KD_ARR 21
+#define KD_FASTOFF 21          /* three absolute offsets of the fast blocks    */
+#define KD_ARR 24
 
 #endif /* XT_PROGRAM_H */
