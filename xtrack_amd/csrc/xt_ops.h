@@ -123,7 +123,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
 
 /* track_magnet_drift.h:91-213 */
 template <class R>
-XT_HD void dipole_quad(PState<R> &p, const double length, const double k0_, const double k1_,
+XT_HD_COLD void dipole_quad(PState<R> &p, const double length, const double k0_, const double k1_,
                        const double h) {
     const R x = p.x, y = p.y, px = p.px, py = p.py, rvv = p.rvv;
     const R delta_plus_1 = p.delta + 1;
