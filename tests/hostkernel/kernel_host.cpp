@@ -29,3 +29,16 @@ extern "C" int32_t xth_track_line(const int32_t *types, const int64_t *offsets,
                            MM, flags);
     return 0;
 }
+
+// div_rcp (reciprocal division used by the kernel) vs IEEE division on
+// n pairs; returns the number of mismatches (bit patterns).
+extern "C" int64_t xth_check_div_rcp(const double *a, const double *b, int64_t n) {
+    int64_t bad = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const double y = 1.0 / b[i];
+        const double q = xt::div_rcp(a[i], b[i], y);
+        const double r = a[i] / b[i];
+        if (__builtin_memcmp(&q, &r, 8) != 0) bad++;
+    }
+    return bad;
+}

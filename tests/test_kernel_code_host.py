@@ -76,3 +76,28 @@ def test_thin_fodo_and_collider_cells():
     _, _, ph, pc = _run(lambda: lattices.collider_ring(n_arcs=1, n_cells=2, k2f=0.03,
                                                        k2d=-0.06), p, num_turns=3)
     assert not compare_exact(ph, pc)
+
+
+def test_reciprocal_division_is_correctly_rounded():
+    """div_rcp(a, b, RN(1/b)) == a/b bit for bit in the range the kernel
+    uses it (|a| in [2^-960, 2^960], |b| in [2^-16, 2^16]), including
+    quotients constructed to lie next to rounding midpoints."""
+    import ctypes
+    from oracle_backend import host_kernel_lib
+    lib = host_kernel_lib()
+    lib.xth_check_div_rcp.restype = ctypes.c_int64
+    lib.xth_check_div_rcp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    rng = np.random.default_rng(0)
+    n = 2_000_000
+    a = rng.uniform(-2, 2, n) * 2.0 ** rng.integers(-900, 900, n)
+    b = rng.uniform(0.5, 2, n) * rng.choice([-1, 1], n) * 2.0 ** rng.integers(-16, 16, n)
+    assert lib.xth_check_div_rcp(a.ctypes.data, b.ctypes.data, n) == 0
+    # near-midpoint quotients: a = (q + ulp/2 * (1 + tiny)) * b with q a double
+    q = rng.uniform(1, 2, n)
+    b2 = rng.uniform(1, 2, n)
+    a2 = (q + np.spacing(q) * 0.5) * b2
+    assert lib.xth_check_div_rcp(a2.ctypes.data, b2.ctypes.data, n) == 0
+    # divisors with all-ones significands (reciprocal next to a power of 2)
+    b3 = np.nextafter(2.0 ** rng.integers(-10, 10, n).astype(float), 0)
+    a3 = rng.uniform(-4, 4, n)
+    assert lib.xth_check_div_rcp(a3.ctypes.data, b3.ctypes.data, n) == 0

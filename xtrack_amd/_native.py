@@ -84,7 +84,8 @@ _lib = None
 
 
 def lib_path():
-    return os.path.join(_PKG, LIB_NAME)
+    """Package copy of the library, or the file named by XTRACK_AMD_LIB."""
+    return os.environ.get('XTRACK_AMD_LIB') or os.path.join(_PKG, LIB_NAME)
 
 
 def load():

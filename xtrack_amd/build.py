@@ -32,17 +32,19 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
-    kern_o = os.path.join(BUILD, 'xt_kernel.o')
-    bld_o = os.path.join(BUILD, 'xt_builder.o')
-    flo_o = os.path.join(BUILD, 'xt_flops.o')
-    _run([HIPCC, *HIP_FLAGS, '-c', os.path.join(CSRC, 'xt_kernel.hip'), '-o', kern_o])
+def build(verbose=True, out=None, extra_hip_flags=(), build_dir=None):
+    BUILD_ = build_dir or BUILD
+    os.makedirs(BUILD_, exist_ok=True)
+    kern_o = os.path.join(BUILD_, 'xt_kernel.o')
+    bld_o = os.path.join(BUILD_, 'xt_builder.o')
+    flo_o = os.path.join(BUILD_, 'xt_flops.o')
+    _run([HIPCC, *HIP_FLAGS, *extra_hip_flags, '-c', os.path.join(CSRC, 'xt_kernel.hip'),
+          '-o', kern_o])
     _run([shutil.which('g++') or 'g++', *CXX_FLAGS, '-c',
           os.path.join(CSRC, 'xt_builder.cpp'), '-o', bld_o])
     _run([shutil.which('g++') or 'g++', *CXX_FLAGS, '-c',
           os.path.join(CSRC, 'xt_flops.cpp'), '-o', flo_o])
-    out = os.path.join(PKG, 'libxtrack_amd.so')
+    out = out or os.path.join(PKG, 'libxtrack_amd.so')
     tmp = out + '.tmp'
     _run([HIPCC, '-shared', f'--offload-arch={ARCH}', '-fPIC', '-o', tmp, kern_o, bld_o, flo_o])
     os.replace(tmp, out)

@@ -102,14 +102,17 @@ bool configure(int64_t model, double k0, double k1, double h, ModelCfg &c) {
     return true;
 }
 
-// constants of one polar drift of length s (track_magnet_drift.h:62-70)
-void polar_consts(double h, double s, double *ca, double *sa, double *t2) {
+// constants of one polar drift of length s (track_magnet_drift.h:62-70);
+// ica = 1/ca, or 0 when |ca| is outside [2^-16, 2^16] (plain divisions then)
+void polar_consts(double h, double s, double *ca, double *sa, double *t2, double *ica) {
     double c, sn, s2, c2;
     xt_sincos(h * s, &sn, &c);
     xt_sincos(0.5 * h * s, &s2, &c2);
     *ca = c;
     *sa = sn;
     *t2 = 2 * s2 * s2;
+    const double ac = std::fabs(c);
+    *ica = (ac >= 0x1p-16 && ac <= 0x1p16) ? 1.0 / c : 0.0;
 }
 
 struct Builder {
@@ -127,9 +130,10 @@ struct Builder {
         case 0: P.op(OP_DRIFT_EXP); P.arg(dl); return true;
         case 1: P.op(OP_DRIFT_EXACT); P.arg(dl); return true;
         case 2: {
-            double ca, sa, t2;
-            polar_consts(h, dl, &ca, &sa, &t2);
+            double ca, sa, t2, ica;
+            polar_consts(h, dl, &ca, &sa, &t2, &ica);
             P.op(OP_POLAR); P.arg(dl); P.arg(1 / h); P.arg(ca); P.arg(sa); P.arg(t2);
+            P.arg(ica);
             return true; }
         case 3:
             P.op(OP_DIPQUAD); P.arg(dl); P.arg(c.k0_drift); P.arg(c.k1_drift); P.arg(h);
@@ -137,15 +141,15 @@ struct Builder {
         case 7: {
             const double a = 0.6756035959798289 * dl;
             const double b = -0.17560359597982889 * dl;
-            double ca, sa, t2, cb, sb, tb;
-            polar_consts(h, a, &ca, &sa, &t2);
-            polar_consts(h, b, &cb, &sb, &tb);
+            double ca, sa, t2, ia, cb, sb, tb, ib;
+            polar_consts(h, a, &ca, &sa, &t2, &ia);
+            polar_consts(h, b, &cb, &sb, &tb, &ib);
             P.op(OP_YOSHIDA4_BEND);
             P.arg(dl); P.arg(1 / h);
             P.arg(1.3512071919596578 * c.k0_drift);
             P.arg((-1.7024143839193155) * c.k0_drift);
-            P.arg(a); P.arg(ca); P.arg(sa); P.arg(t2);
-            P.arg(b); P.arg(cb); P.arg(sb); P.arg(tb);
+            P.arg(a); P.arg(ca); P.arg(sa); P.arg(t2); P.arg(ia);
+            P.arg(b); P.arg(cb); P.arg(sb); P.arg(tb); P.arg(ib);
             return true; }
         case 8: {
             static const double d[4] = {3.922568052387799819591407413100e-01,
@@ -160,9 +164,9 @@ struct Builder {
             P.arg(dl); P.arg(1 / h);
             for (int i = 0; i < 4; i++) P.arg(k[i] * c.k0_drift);
             for (int i = 0; i < 4; i++) {
-                double s = d[i] * dl, cc, ss, tt;
-                polar_consts(h, s, &cc, &ss, &tt);
-                P.arg(s); P.arg(cc); P.arg(ss); P.arg(tt);
+                double s = d[i] * dl, cc, ss, tt, ii;
+                polar_consts(h, s, &cc, &ss, &tt, &ii);
+                P.arg(s); P.arg(cc); P.arg(ss); P.arg(tt); P.arg(ii);
             }
             return true; }
         default:

@@ -30,7 +30,14 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-__global__ __launch_bounds__(BLOCK) void track_kernel(
+/* optional occupancy target (waves per SIMD) for register-budget sweeps */
+#ifdef XTM_WAVES_PER_EU
+#define XTM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(XTM_WAVES_PER_EU)))
+#else
+#define XTM_OCCUPANCY
+#endif
+
+__global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     const int64_t *__restrict__ entry, const double *__restrict__ prog, double line_length,
     xtm_particles P, int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
     int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor, xtm_monitor M,

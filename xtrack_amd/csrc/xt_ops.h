@@ -52,6 +52,9 @@ template <class R>
 struct PState {
     R x, px, y, py, zeta, delta, ptau, rpp, rvv, s, ax, ay;
     R chi, charge_ratio, beta0, p0c;
+    /* 1./rvv, kept equal to that IEEE quotient: set when the particle is
+       loaded and whenever rvv changes (update_ptau) */
+    R inv_rvv;
     int64_t at_element, at_turn, state;
 };
 
@@ -66,12 +69,40 @@ XT_HD double xfabs(double v) { return XT_FABS(v); }
 XT_HD bool xfinite(double v) { return XT_FABS(v) < XT_INF; }
 XT_HD double xval(double v) { return v; }
 
+/* a/b given y = RN(1/b): Markstein's correction applied twice,
+   q0 = RN(a y), r = a - b q (exact with fma), q' = RN(q + r y).  After the
+   first step q is within one ulp of a/b; the second step then yields the
+   correctly rounded quotient (P. Markstein, IBM J. Res. Dev. 34 (1990);
+   J.-M. Muller et al., Handbook of Floating-Point Arithmetic, 2nd ed.,
+   section 4.9).  Valid when a, b, y are finite, b normal and no
+   intermediate over/underflows; callers check ranges (div_ok). */
+XT_HD double div_rcp(double a, double b, double y) {
+    double q = a * y;
+    double r = XT_FMA(-q, b, a);
+    q = XT_FMA(r, y, q);
+    r = XT_FMA(-q, b, a);
+    return XT_FMA(r, y, q);
+}
+
+XT_HD double xdiv_rcp(double a, double b, double y) { return div_rcp(a, b, y); }
+
+/* |v| in [2^-960, 2^960] or v == 0 */
+XT_HD bool num_ok(double v) {
+    const double a = XT_FABS(v);
+    return (a >= 0x1p-960 && a <= 0x1p960) || v == 0.0;
+}
+/* |v| in [2^-16, 2^16] */
+XT_HD bool den_ok(double v) {
+    const double a = XT_FABS(v);
+    return a >= 0x1p-16 && a <= 0x1p16;
+}
+
 /* ---- drift maps --------------------------------------------------------- */
 /* track_drift.h:11-23 */
 template <class R>
 XT_HD void drift_expanded(PState<R> &p, const double length) {
     R const rpp = p.rpp;
-    R const rv0v = 1. / p.rvv;
+    R const rv0v = p.inv_rvv;
     R const xp = p.px * rpp;
     R const yp = p.py * rpp;
     R const dzeta = 1 - rv0v * (1. + (xp * xp + yp * yp) / 2.);
@@ -85,7 +116,7 @@ XT_HD void drift_expanded(PState<R> &p, const double length) {
 template <class R>
 XT_HD void drift_exact(PState<R> &p, const double length) {
     R const px = p.px, py = p.py;
-    R const rv0v = 1. / p.rvv;
+    R const rv0v = p.inv_rvv;
     R const one_plus_delta = 1. + p.delta;
     R const one_over_pz = 1. / xsqrt(one_plus_delta * one_plus_delta - px * px - py * py);
     R const dzeta = 1 - rv0v * one_plus_delta * one_over_pz;
@@ -96,10 +127,15 @@ XT_HD void drift_exact(PState<R> &p, const double length) {
 }
 
 /* track_magnet_drift.h:45-87 with rho = 1/h, ca = cos(h s), sa = sin(h s),
-   t2 = 2 sin(h s / 2)^2 from the element program */
+   t2 = 2 sin(h s / 2)^2 and ica = 1/ca (0 when ca is out of the range of
+   the reciprocal division) from the element program.  The quotients by
+   ca, pz and rvv use div_rcp with the reciprocals 1/ca, _pz = 1/pz and
+   inv_rvv, all IEEE-rounded reciprocals, which gives the same bits as the
+   division; lanes outside div_rcp's validity range recompute the path
+   length with plain divisions. */
 template <class R>
 XT_HD void polar_drift(PState<R> &p, const double length, const double rho, const double ca,
-                       const double sa, const double t2) {
+                       const double sa, const double t2, const double ica) {
     const R rvv = p.rvv;
     const R x = p.x, y = p.y, px = p.px, py = p.py;
     const R one_plus_delta = p.delta + 1.0;
@@ -113,11 +149,24 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     const R new_x = (x + rho * (t2 + sapxt)) * _ptt;
     const R new_px = ca * px + sa * pz;
     const R new_y = y + pst * py;
-    const R delta_ell = one_plus_delta * xr * sa / ca / pz / (1 - px * sa / ca / pz);
+    /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
+    const R n1 = one_plus_delta * xr * sa;
+    const R n3 = px * sa;
+    const R q2 = xdiv_rcp(xdiv_rcp(n1, R(ca), ica), pz, _pz);
+    const R q4 = xdiv_rcp(xdiv_rcp(n3, R(ca), ica), pz, _pz);
+    const R den = 1 - q4;
+    const R delta_ell = q2 / den;
+    R dz = length - xdiv_rcp(delta_ell, rvv, p.inv_rvv);
+    const bool ok = (ica != 0.0) && num_ok(xval(n1)) && num_ok(xval(n3)) &&
+                    den_ok(xval(pz)) && den_ok(xval(rvv)) && den_ok(xval(den));
+    if (!ok) {
+        const R del = n1 / ca / pz / (1 - n3 / ca / pz);
+        dz = length - del / rvv;
+    }
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;
-    p.zeta = p.zeta + (length - delta_ell / rvv);
+    p.zeta = p.zeta + dz;
     p.s = p.s + length;
 }
 
@@ -248,7 +297,7 @@ XT_HD_COLD void magnet_kick_literal(PState<R> &p, double kick_weight, const doub
     if (wbits(d[KD_ROT_FRAME])) {
         double const hl = d[KD_H] * length * kick_weight + d[KD_HXL] * kick_weight;
         dpx = dpx + hl * (1. + p.delta);
-        R const rv0v = 1. / p.rvv;
+        R const rv0v = p.inv_rvv;
         dzeta = dzeta + -rv0v * hl * x;
     }
     const double htot = d[KD_HTOT];
@@ -320,7 +369,7 @@ XT_HD void magnet_kick(PState<R> &p, const double *__restrict__ a, const double 
             const double hl = a[3];
             dpx = hl * (1. + p.delta);
             hx = true;
-            R const rv0v = 1. / p.rvv;
+            R const rv0v = p.inv_rvv;
             dzeta = -rv0v * hl * x;
             hz = true;
         }
@@ -357,6 +406,7 @@ XT_HD void update_ptau(PState<R> &p, R new_ptau) {
     p.rvv = new_rvv;
     p.ptau = ptau;
     p.rpp = new_rpp;
+    p.inv_rvv = 1. / new_rvv;
 }
 
 template <class R>
@@ -412,8 +462,8 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             pc += 2;
             break;
         case OP_POLAR:
-            polar_drift(p, a[0], a[1], a[2], a[3], a[4]);
-            pc += 6;
+            polar_drift(p, a[0], a[1], a[2], a[3], a[4], a[5]);
+            pc += 7;
             break;
         case OP_DIPQUAD:
             dipole_quad(p, a[0], a[1], a[2], a[3]);
@@ -422,14 +472,14 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
         case OP_YOSHIDA4_BEND: {
             /* track_magnet_drift.h:500-509 */
             const double L = a[0], rho = a[1], c1 = a[2], c2 = a[3];
-            polar_drift(p, a[4], rho, a[5], a[6], a[7]);
+            polar_drift(p, a[4], rho, a[5], a[6], a[7], a[8]);
             p.px = p.px - c1 * p.chi * L;
-            polar_drift(p, a[8], rho, a[9], a[10], a[11]);
+            polar_drift(p, a[9], rho, a[10], a[11], a[12], a[13]);
             p.px = p.px - c2 * p.chi * L;
-            polar_drift(p, a[8], rho, a[9], a[10], a[11]);
+            polar_drift(p, a[9], rho, a[10], a[11], a[12], a[13]);
             p.px = p.px - c1 * p.chi * L;
-            polar_drift(p, a[4], rho, a[5], a[6], a[7]);
-            pc += 13;
+            polar_drift(p, a[4], rho, a[5], a[6], a[7], a[8]);
+            pc += 15;
             break;
         }
         case OP_YOSHIDA6_BEND: {
@@ -439,14 +489,14 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             const double *dd = a + 6;
             for (int j = 0; j < 8; j++) {
                 const int di = j < 4 ? j : 7 - j;
-                const double *s = dd + 4 * di;
-                polar_drift(p, s[0], rho, s[1], s[2], s[3]);
+                const double *s = dd + 5 * di;
+                polar_drift(p, s[0], rho, s[1], s[2], s[3], s[4]);
                 if (j < 7) {
                     const int ki = j < 4 ? j : 6 - j;
                     p.px = p.px - kc[ki] * p.chi * L;
                 }
             }
-            pc += 23;
+            pc += 27;
             break;
         }
         case OP_KICK:
@@ -586,6 +636,7 @@ XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__re
     p.chi = P.chi[i]; p.charge_ratio = P.charge_ratio[i];
     p.beta0 = P.beta0[i]; p.p0c = P.p0c[i];
     p.at_element = P.at_element[i]; p.at_turn = P.at_turn[i]; p.state = P.state[i];
+    p.inv_rvv = 1. / p.rvv;
     const int64_t pid = flag_monitor ? P.particle_id[i] : 0;
     const double q0 = P.q0, t_sim = P.t_sim;
     const int64_t ele_stop = (int64_t)ele_start + num_ele_track;

@@ -25,10 +25,10 @@ enum xt_op {
     OP_END = 0,
     OP_DRIFT_EXP = 1,      /* L                                               */
     OP_DRIFT_EXACT = 2,    /* L                                               */
-    OP_POLAR = 3,          /* L rho ca sa t2      (t2 = 2*sa2*sa2)            */
+    OP_POLAR = 3,          /* L rho ca sa t2 ica  (t2 = 2*sa2*sa2, ica = 1/ca) */
     OP_DIPQUAD = 4,        /* L k0 k1 h                                        */
-    OP_YOSHIDA4_BEND = 5,  /* L rho c1k0 c2k0 | a ca sa t2 | b ca sa t2       */
-    OP_YOSHIDA6_BEND = 6,  /* L rho kc0..kc3 | 4 x (s ca sa t2)               */
+    OP_YOSHIDA4_BEND = 5,  /* L rho c1k0 c2k0 | a ca sa t2 ica | b ...        */
+    OP_YOSHIDA6_BEND = 6,  /* L rho kc0..kc3 | 4 x (s ca sa t2 ica)           */
     OP_KICK = 7,           /* w desc mask hl g0 h1                            */
     OP_EDGE_AXAY = 8,      /* ca cb   (ax = ca*(y - 0), ay = cb*(x - 0))       */
     OP_EDGE_CLEAR_AXAY = 9,/*                                                  */
@@ -42,7 +42,7 @@ enum xt_op {
 
 /* number of constants following each op word */
 #define XT_NARGS_KICK 6
-#define XT_OP_NARGS_INIT {0, 1, 1, 5, 4, 12, 22, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0}
+#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0}
 
 /* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
  * OP_KICK through an absolute offset into prog:
