@@ -152,6 +152,11 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
     const R n1 = one_plus_delta * xr * sa;
     const R n3 = px * sa;
+#ifdef XTM_PLAIN_DIVISION
+    const R delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
+    const R dz = length - delta_ell / rvv;
+    (void)ica;
+#else
     const R q2 = xdiv_rcp(xdiv_rcp(n1, R(ca), ica), pz, _pz);
     const R q4 = xdiv_rcp(xdiv_rcp(n3, R(ca), ica), pz, _pz);
     const R den = 1 - q4;
@@ -163,6 +168,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
         const R del = n1 / ca / pz / (1 - n3 / ca / pz);
         dz = length - del / rvv;
     }
+#endif
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;
@@ -172,8 +178,8 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
 
 /* track_magnet_drift.h:91-213 */
 template <class R>
-XT_HD_COLD void dipole_quad(PState<R> &p, const double length, const double k0_, const double k1_,
-                       const double h) {
+XT_HD_COLD PState<R> dipole_quad(PState<R> p, const double length, const double k0_,
+                                 const double k1_, const double h) {
     const R x = p.x, y = p.y, px = p.px, py = p.py, rvv = p.rvv;
     const R delta_plus_1 = p.delta + 1;
     const R chi = p.chi;
@@ -250,6 +256,7 @@ XT_HD_COLD void dipole_quad(PState<R> &p, const double length, const double k0_,
     p.py = py_;
     p.zeta = p.zeta + dzeta;
     p.s = p.s + length;
+    return p;
 }
 
 /* ---- multipole kick (track_magnet_kick.h:24-144, 183-228) ----------------- */
@@ -278,7 +285,8 @@ XT_HD void kick_pass(PState<R> &p, int64_t order, const double *__restrict__ f,
 }
 
 template <class R>
-XT_HD_COLD void magnet_kick_literal(PState<R> &p, double kick_weight, const double *__restrict__ d) {
+XT_HD_COLD PState<R> magnet_kick_literal(PState<R> p, double kick_weight,
+                                         const double *__restrict__ d) {
     const R chi = p.chi, x = p.x, y = p.y;
     const int64_t order = wbits(d[KD_ORDER]);
     const int64_t order_rel = wbits(d[KD_ORDER_REL]);
@@ -309,6 +317,7 @@ XT_HD_COLD void magnet_kick_literal(PState<R> &p, double kick_weight, const doub
     p.px = p.px + dpx;
     p.py = p.py + dpy;
     p.zeta = p.zeta + dzeta;
+    return p;
 }
 
 /* Horner pass on precomputed coefficients a[k] = (knl[k]*factor)*f[k],
@@ -339,7 +348,7 @@ XT_HD void magnet_kick(PState<R> &p, const double *__restrict__ a, const double 
     const bool sane = (xval(p.chi) == 1.0) && xfinite(xval(p.x)) && xfinite(xval(p.y)) &&
                       xfinite(xval(p.delta)) && xfinite(xval(p.rvv)) && (xval(p.rvv) != 0.0);
     if (!sane) {
-        magnet_kick_literal(p, w, d);
+        p = magnet_kick_literal(p, w, d);
         return;
     }
     const R x = p.x, y = p.y;
@@ -410,7 +419,7 @@ XT_HD void update_ptau(PState<R> &p, R new_ptau) {
 }
 
 template <class R>
-XT_HD_COLD void rf_kick(PState<R> &p, const double *__restrict__ a, double line_length,
+XT_HD_COLD PState<R> rf_kick(PState<R> p, const double *__restrict__ a, double line_length,
                         double q0, double t_sim) {
     const double voltage = a[0];
     const double frequency0 = a[1];
@@ -438,6 +447,7 @@ XT_HD_COLD void rf_kick(PState<R> &p, const double *__restrict__ a, double line_
     R const mass_ratio = p.charge_ratio / p.chi;
     ptau = ptau + (energy_kick + rfmultipole_energy_kick) / p.p0c / mass_ratio;
     update_ptau(p, ptau);
+    return p;
 }
 
 /* ---- element program interpreter ------------------------------------------ */
@@ -466,7 +476,7 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             pc += 7;
             break;
         case OP_DIPQUAD:
-            dipole_quad(p, a[0], a[1], a[2], a[3]);
+            p = dipole_quad(p, a[0], a[1], a[2], a[3]);
             pc += 5;
             break;
         case OP_YOSHIDA4_BEND: {
@@ -521,7 +531,7 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             break;
         }
         case OP_RF_KICK:
-            if (!kill_cavity) rf_kick(p, a, line_length, q0, t_sim);
+            if (!kill_cavity) p = rf_kick(p, a, line_length, q0, t_sim);
             pc += 7;
             break;
         case OP_LIMIT_RECT:
@@ -594,7 +604,7 @@ XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i
 }
 
 XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int64_t i,
-                               const PState<double> &p, int64_t pid) {
+                               const PState<double> p, int64_t pid) {
     const int64_t n_turns_record = M.stop_at_turn - M.start_at_turn;
     const int64_t at_turn = M.ebe_mode ? p.at_element : p.at_turn;
     if (M.n_repetitions == 1) {
