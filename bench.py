@@ -14,12 +14,21 @@ with an operation-counting scalar, over the kernel time measured with HIP
 events on the launch stream) and a CPU baseline: the CPU restatement of
 the reference tracker (oracle/, platform libm, OpenMP with the reference's
 chunking) timed on a bounded sample of the same workload on this host.
+
+HBM traffic of the tracking kernel is measured by running this script twice
+more under ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE) as child
+processes before this process touches the GPU; each child also runs a plain
+copy of known size, whose counter readings convert the counters to bytes.
 """
 import argparse
-import ctypes
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -35,6 +44,11 @@ NEMITT = 3.75e-6
 SIGMA_Z = 0.0755
 SIGMA_DELTA = 1.13e-4
 FLOP_PER_CALL = 20     # weight of one elementary-function call
+# particle state moved per particle and launch: 19 fields of 8 bytes read
+# (x px y py zeta delta ptau rpp rvv s ax ay chi charge_ratio beta0 p0c
+# at_element at_turn state), 15 written back (all but chi charge_ratio beta0 p0c)
+STATE_BYTES_PER_PARTICLE = (19 + 15) * 8
+COPY_MIB = 256
 
 
 def parse():
@@ -44,11 +58,76 @@ def parse():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--turns-per-step', type=int, default=2)
     ap.add_argument('--particles', type=int, default=1_000_000)
-    ap.add_argument('--cpu-sample-particles', type=int, default=2048)
-    ap.add_argument('--cpu-sample-turns', type=int, default=10)
+    ap.add_argument('--cpu-sample-particles', type=int, default=8192)
+    ap.add_argument('--cpu-sample-turns', type=int, default=20)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--traffic', choices=('measure', 'none'), default='measure',
+                    help='measure HBM traffic with two rocprofv3 --pmc child runs')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run a plain device copy of known size before tracking '
+                         '(used by the traffic child runs)')
     ap.add_argument('--rematch', action='store_true')
     return ap.parse_args()
+
+
+def _read_counter(outdir, counter):
+    """(copy_value, track_value) of `counter` from a rocprofv3 csv run: the
+    copy is the calibration dispatch (kernel name holds 'neg'), the tracking
+    value is that of the last track_kernel dispatch (the timed launch)."""
+    files = glob.glob(os.path.join(outdir, '**', '*counter_collection.csv'), recursive=True)
+    if not files:
+        raise RuntimeError(f'no counter_collection.csv under {outdir}')
+    per_dispatch = {}
+    for fn in files:
+        with open(fn, newline='') as fh:
+            for row in csv.DictReader(fh):
+                if row.get('Counter_Name') != counter:
+                    continue
+                key = (fn, int(row['Dispatch_Id']))
+                name = row['Kernel_Name']
+                val = float(row['Counter_Value'])
+                old = per_dispatch.get(key, (name, 0.0))
+                per_dispatch[key] = (name, old[1] + val)
+    items = sorted(per_dispatch.items(), key=lambda kv: kv[0][1])
+    copy = [v for _, (nm, v) in items if 'neg' in nm]
+    track = [v for _, (nm, v) in items if 'track_kernel' in nm]
+    if not copy or not track:
+        raise RuntimeError(f'{counter}: calibration or tracking dispatch not found')
+    return max(copy), track[-1]
+
+
+def measure_traffic(args):
+    """HBM bytes per tracking launch from two rocprofv3 --pmc child runs
+    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by the
+    reading of the same counter for a copy of COPY_MIB MiB (reads and writes
+    exactly that many bytes)."""
+    prof = shutil.which('rocprofv3')
+    if prof is None:
+        raise RuntimeError('rocprofv3 not on PATH')
+    copy_bytes = COPY_MIB * 2 ** 20
+    res = {}
+    tmp = tempfile.mkdtemp(prefix='xtm_traffic_')
+    try:
+        for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+            outdir = os.path.join(tmp, counter)
+            cmd = [prof, '--pmc', counter, '--output-format', 'csv', '-d', outdir, '-o', 'run',
+                   '--', sys.executable, os.path.abspath(__file__), '--steps', '1',
+                   '--warmup', '0', '--turns-per-step', str(args.turns_per_step),
+                   '--particles', str(args.particles), '--no-cpu-baseline',
+                   '--traffic', 'none', '--copy-calibration']
+            print(f'bench.py: traffic pass {counter}', file=sys.stderr, flush=True)
+            with open(os.path.join(tmp, counter + '.log'), 'w') as log:
+                subprocess.run(cmd, stdout=log, stderr=subprocess.STDOUT, timeout=420,
+                               check=True)
+            kib_copy, kib_track = _read_counter(outdir, counter)
+            scale = copy_bytes / (kib_copy * 1024.0)
+            res[counter] = {'kib_track': kib_track, 'kib_copy': kib_copy, 'scale': scale,
+                            'bytes': kib_track * 1024.0 * scale}
+            print(f'bench.py: {counter} track {kib_track:.0f} KiB, copy {kib_copy:.0f} KiB '
+                  f'(scale {scale:.4f})', file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return res
 
 
 def cpu_baseline(line_fn, tw, n, turns):
@@ -80,6 +159,13 @@ def main():
               'particles shard across GPUs with no exchange during tracking, '
               'so N GPUs give N times the per-GPU rate.', file=sys.stderr)
         sys.exit(2)
+    traffic = None
+    if args.traffic == 'measure':
+        try:
+            traffic = measure_traffic(args)
+        except Exception as exc:      # counters unavailable: report null traffic
+            print(f'bench.py: traffic not measured ({exc})', file=sys.stderr, flush=True)
+
     import torch
     from xtrack_amd import lattices, optics, _native
 
@@ -109,6 +195,13 @@ def main():
                                          ops.ctypes.data))
     flop_pt = ops[5] + FLOP_PER_CALL * ops[4]
 
+    if args.copy_calibration:
+        src = torch.ones(COPY_MIB * 2 ** 20 // 8, dtype=torch.float64, device='cuda:0')
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+
     tps = args.turns_per_step
     for _ in range(args.warmup):
         line.track(parts, num_turns=tps)
@@ -134,6 +227,10 @@ def main():
     # per launch: all particles alive at the start of the step x tps turns
     kern_s = float(np.mean(kern_ms)) / 1e3
     achieved = flop_pt * (pturns / args.steps) / kern_s / 1e12
+    alg_bytes = STATE_BYTES_PER_PARTICLE * args.particles
+    traffic_bytes = None
+    if traffic is not None:
+        traffic_bytes = traffic['FETCH_SIZE']['bytes'] + traffic['WRITE_SIZE']['bytes']
     out = {
         'metric': 'particle-turns/s (FP64) on 1 MI355X',
         'value': value,
@@ -153,8 +250,13 @@ def main():
                    'particles_alive_end': n_alive,
                    'tunes': [round(float(tw['qx']), 4), round(float(tw['qy']), 4)],
                    'chromaticity': [round(float(chroma[0]), 3), round(float(chroma[1]), 3)]},
-        'roofline': {'bound': 'fp64-valu', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS,
-                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None,
+        'roofline': {'bound': 'mfma', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
+                                                      '78.6 TFLOP/s, vector and matrix)',
+                     'achieved': achieved, 'peak': PEAK_FP64_TFLOPS,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP64_TFLOPS,
+                     'traffic': traffic_bytes,
+                     'algorithmic_bytes_per_launch': alg_bytes,
+                     'traffic_counters': traffic,
                      'flop_per_particle_turn': flop_pt,
                      'ops_per_particle_turn': {'add': ops[0], 'mul': ops[1], 'div': ops[2],
                                                'sqrt': ops[3], 'calls': ops[4]},

@@ -50,15 +50,49 @@ def random_particles(n, seed=0, scale=1e-3, p0c=7e9, spread=1.0):
                         zeta=rng.normal(0, 0.05, n), delta=rng.normal(0, 1e-3, n))
 
 
+def guard_edge_particles(n, seed=0, p0c=7e9):
+    """Beam-like particles mixed with values on both sides of the ranges in
+    which the kernel replaces divisions by reciprocal multiplications:
+    px = +0, -0, subnormal and around 2^-900; pz driven below 2^-16 by a
+    large |py| or by delta close to -1; very low energy (small rvv); |x|,
+    |y| large enough for x*x to overflow; coordinates of +0 and -0."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1e-3, n)
+    px = rng.normal(0, 1e-4, n)
+    y = rng.normal(0, 1e-3, n)
+    py = rng.normal(0, 1e-4, n)
+    delta = rng.normal(0, 1e-3, n)
+    k = n // 12
+    px[0 * k:1 * k] = 0.0
+    px[1 * k:2 * k] = -0.0
+    px[2 * k:3 * k] = 5e-324 * rng.choice([-1, 1], k)
+    px[3 * k:4 * k] = 2.0 ** -899 * rng.uniform(0.5, 2, k) * rng.choice([-1, 1], k)
+    px[4 * k:5 * k] = 2.0 ** -901 * rng.uniform(0.5, 2, k)
+    py[5 * k:6 * k] = (1 - 10.0 ** rng.uniform(-13, -8, k)) * rng.choice([-1, 1], k)
+    px[5 * k:6 * k] = 0.0
+    delta[6 * k:7 * k] = -1 + 10.0 ** rng.uniform(-7, -3, k)
+    delta[7 * k:8 * k] = -1 + 10.0 ** rng.uniform(-12, -8, k)
+    x[8 * k:9 * k] = rng.uniform(-0.9, 0.9, k)
+    px[9 * k:10 * k] = rng.uniform(-0.3, 0.3, k)
+    x[10 * k:11 * k] = 10.0 ** rng.uniform(150, 300, k) * rng.choice([-1, 1], k)
+    y[10 * k:11 * k] = 10.0 ** rng.uniform(150, 300, k)
+    zeta = rng.normal(0, 0.05, n)
+    for arr in (x, y, px, py, zeta):
+        arr[11 * k:12 * k] = rng.choice([0.0, -0.0], k)
+    return xt.Particles(p0c=p0c, x=x, px=px, y=y, py=py, delta=delta, zeta=zeta)
+
+
 def compare_exact(a, b, fields=COMPARED):
     """a, b: Particles.  Returns list of (field, n_mismatch, example) for
-    fields whose values differ (IEEE equality; NaN == NaN; +0 == -0)."""
+    fields whose values differ: floating-point values must have the same
+    bits (the sign of zero included); NaNs match NaNs whatever their payload."""
     sa, sb = a.sorted_by_id(), b.sorted_by_id()
     bad = []
     for ff in fields:
         va, vb = sa[ff], sb[ff]
         if va.dtype.kind == 'f':
-            eq = (va == vb) | (np.isnan(va) & np.isnan(vb))
+            eq = (((va == vb) & (np.signbit(va) == np.signbit(vb)))
+                  | (np.isnan(va) & np.isnan(vb)))
         else:
             eq = va == vb
         if not eq.all():

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import xtrack_amd as xt
-from helpers import mixed_line, random_particles, compare_exact
+from helpers import mixed_line, random_particles, compare_exact, guard_edge_particles
 from oracle_backend import OracleBackend
 
 pytestmark = pytest.mark.gpu
@@ -32,6 +32,19 @@ def test_mixed_ring_bit_exact(n_turns):
     n_lost = int((pc.sorted_by_id()['state'] <= 0).sum())
     print(f'lost={n_lost}', flush=True)
     assert not bad, bad
+
+
+def test_reciprocal_division_range_edges():
+    """Lanes on both sides of the reciprocal-division ranges, mixed within
+    waves with ordinary lanes."""
+    print('[division range edges]', flush=True)
+    for seed, kw in ((4, dict(num_turns=2)), (5, dict(num_elements=7))):
+        p = guard_edge_particles(3000, seed=seed)
+        lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=2, aperture=0.95), p)
+        lg.track(pg, **kw)
+        lc.track(pc, **kw)
+        bad = compare_exact(pg, pc)
+        assert not bad, bad
 
 
 def test_losses_on_every_element_kind():

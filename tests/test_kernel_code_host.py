@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import xtrack_amd as xt
-from helpers import mixed_line, random_particles, compare_exact
+from helpers import mixed_line, random_particles, compare_exact, guard_edge_particles
 from oracle_backend import OracleBackend, HostKernelBackend
 
 
@@ -55,6 +55,18 @@ def test_nonfinite_and_extreme():
     assert set(np.unique(pc.sorted_by_id()['state'])) >= {-1, 0}
 
 
+def test_reciprocal_division_range_edges():
+    """Lanes on both sides of the reciprocal-division ranges (px = +-0,
+    subnormal, ~2^-900; pz < 2^-16; delta near -1) take the literal form
+    where needed and match the oracle bit for bit."""
+    p = guard_edge_particles(1200, seed=4)
+    _, _, ph, pc = _run(lambda: mixed_line(n_cells=2, aperture=0.95), p, num_turns=2)
+    assert not compare_exact(ph, pc)
+    p = guard_edge_particles(1200, seed=5)
+    _, _, ph, pc = _run(lambda: mixed_line(n_cells=2, aperture=0.95), p, num_elements=7)
+    assert not compare_exact(ph, pc)
+
+
 @pytest.mark.parametrize('mode', [True, 'ONE_TURN_EBE'])
 def test_monitors(mode):
     p = random_particles(200, seed=5, scale=3e-3)
@@ -79,8 +91,9 @@ def test_thin_fodo_and_collider_cells():
 
 
 def test_reciprocal_division_is_correctly_rounded():
-    """div_rcp(a, b, RN(1/b)) == a/b bit for bit in the range the kernel
-    uses it (|a| in [2^-960, 2^960], |b| in [2^-16, 2^16]), including
+    """div_rcp(a, b, RN(1/b)) == a/b bit for bit in a range wider than the
+    kernel uses (dividends within [2^-948, 2^948], |b| in [2^-16, 2^16]):
+    |a| in [2^-960, 2^960], including
     quotients constructed to lie next to rounding midpoints."""
     import ctypes
     from oracle_backend import host_kernel_lib

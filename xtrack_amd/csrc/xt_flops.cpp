@@ -66,7 +66,6 @@ inline void xsinhcosh(FC v, FC *s, FC *c) {
 }
 inline FC xsin(FC v) { g.trans += 1; return FC(xt_sin(v.v)); }
 inline double xval(FC v) { return v.v; }
-inline FC xdiv_rcp(FC a, FC b, FC) { g.div++; return FC(a.v / b.v); }
 }  // namespace xt
 
 #include "xt_ops.h"
@@ -91,6 +90,7 @@ int xt_count_turn(const std::vector<int64_t> &entry, const std::vector<double> &
     p.chi = 1.0; p.charge_ratio = 1.0; p.beta0 = beta0; p.p0c = p0c;
     p.inv_rvv = FC(1.0) / p.rvv;
     p.at_element = 0; p.at_turn = 0; p.state = 1;
+    xt::refresh_flags(p);
     g = Counts();
     for (size_t e = 0; e < entry.size(); e++) {
         xt::run_element(p, prog.data(), entry[e], 0, line_length, q0, 0.0);

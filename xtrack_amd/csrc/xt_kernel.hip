@@ -30,8 +30,14 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-/* optional occupancy target (waves per SIMD) for register-budget sweeps */
-#ifdef XTM_WAVES_PER_EU
+/* Occupancy target (waves per SIMD).  Four waves cap the tracking kernel
+   at 128 VGPRs; on MI355X this beat both the unconstrained allocation
+   (~168 VGPRs, 3 waves) and 5 waves (heavy spilling) on the thick ring.
+   0 leaves the allocation to the compiler. */
+#ifndef XTM_WAVES_PER_EU
+#define XTM_WAVES_PER_EU 4
+#endif
+#if XTM_WAVES_PER_EU > 0
 #define XTM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(XTM_WAVES_PER_EU)))
 #else
 #define XTM_OCCUPANCY

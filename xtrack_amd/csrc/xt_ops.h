@@ -56,6 +56,11 @@ struct PState {
        loaded and whenever rvv changes (update_ptau) */
     R inv_rvv;
     int64_t at_element, at_turn, state;
+    /* per-particle predicates of quantities that change only at load and in
+       update_ptau (refresh_flags): the folded multipole kick may be used
+       (chi == 1, finite delta, rvv and 1/rvv, rvv != 0); rvv is in the
+       range of the reciprocal division (den_ok) */
+    bool fast_ok, rvv_ok;
 };
 
 XT_HD int64_t wbits(double d) { int64_t u; __builtin_memcpy(&u, &d, 8); return u; }
@@ -84,17 +89,23 @@ XT_HD double div_rcp(double a, double b, double y) {
     return XT_FMA(r, y, q);
 }
 
-XT_HD double xdiv_rcp(double a, double b, double y) { return div_rcp(a, b, y); }
 
-/* |v| in [2^-960, 2^960] or v == 0 */
+/* |v| in [2^-900, 2^900] (zero and non-finite values excluded) */
 XT_HD bool num_ok(double v) {
     const double a = XT_FABS(v);
-    return (a >= 0x1p-960 && a <= 0x1p960) || v == 0.0;
+    return a >= 0x1p-900 && a <= 0x1p900;
 }
 /* |v| in [2^-16, 2^16] */
 XT_HD bool den_ok(double v) {
     const double a = XT_FABS(v);
     return a >= 0x1p-16 && a <= 0x1p16;
+}
+
+template <class R>
+XT_HD void refresh_flags(PState<R> &p) {
+    p.fast_ok = (xval(p.chi) == 1.0) && xfinite(xval(p.delta)) && xfinite(xval(p.rvv)) &&
+                (xval(p.rvv) != 0.0) && xfinite(xval(p.inv_rvv));
+    p.rvv_ok = den_ok(xval(p.rvv));
 }
 
 /* ---- drift maps --------------------------------------------------------- */
@@ -126,13 +137,54 @@ XT_HD void drift_exact(PState<R> &p, const double length) {
     p.s = p.s + length;
 }
 
+/* Path-length term of the polar drift, the reference's literal divisions:
+   length - (n1/ca/pz/(1 - n3/ca/pz))/rvv.  Out of line so that the
+   reciprocal-division path below cannot be if-converted into computing
+   both forms on every lane. */
+XT_HD_COLD double polar_dz_plain(double n1, double n3, double ca, double pz, double rvv,
+                                 double length) {
+    const double delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
+    return length - delta_ell / rvv;
+}
+
+/* Same value through div_rcp with the reciprocals ica = RN(1/ca) (element
+   program), _pz = RN(1/pz) and inv_rvv = RN(1/rvv), which the drift has at
+   hand.  Range argument for exactness: with |n1|,|n3| in [2^-900, 2^900]
+   and ca, pz, den, rvv in [2^-16, 2^16] every dividend is within
+   [2^-948, 2^948], far from over- and underflow of the remainder.  Lanes
+   outside those ranges (including n3 == 0 for px == 0) take the literal
+   form. */
+XT_HD double polar_dz(double n1, double n3, double ca, double ica, double pz, double _pz,
+                      double rvv, double inv_rvv, bool rvv_ok, double length) {
+#ifdef XTM_PLAIN_DIVISION
+    (void)ica; (void)_pz; (void)inv_rvv; (void)rvv_ok;
+    const double delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
+    return length - delta_ell / rvv;
+#else
+    const double q2 = div_rcp(div_rcp(n1, ca, ica), pz, _pz);
+    const double q4 = div_rcp(div_rcp(n3, ca, ica), pz, _pz);
+    const double den = 1 - q4;
+    const double delta_ell = q2 / den;
+    double dz = length - div_rcp(delta_ell, rvv, inv_rvv);
+    const bool ok = rvv_ok && (ica != 0.0) && num_ok(n1) && num_ok(n3) && den_ok(pz) &&
+                    den_ok(den);
+    if (!ok) dz = polar_dz_plain(n1, n3, ca, pz, rvv, length);
+    return dz;
+#endif
+}
+
+/* operation-counting and other scalar types: the literal form */
+template <class R>
+XT_HD R polar_dz(R n1, R n3, double ca, double ica, R pz, R _pz, R rvv, R inv_rvv,
+                 bool rvv_ok, double length) {
+    (void)ica; (void)_pz; (void)inv_rvv; (void)rvv_ok;
+    const R delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
+    return length - delta_ell / rvv;
+}
+
 /* track_magnet_drift.h:45-87 with rho = 1/h, ca = cos(h s), sa = sin(h s),
-   t2 = 2 sin(h s / 2)^2 and ica = 1/ca (0 when ca is out of the range of
-   the reciprocal division) from the element program.  The quotients by
-   ca, pz and rvv use div_rcp with the reciprocals 1/ca, _pz = 1/pz and
-   inv_rvv, all IEEE-rounded reciprocals, which gives the same bits as the
-   division; lanes outside div_rcp's validity range recompute the path
-   length with plain divisions. */
+   t2 = 2 sin(h s / 2)^2 and ica = 1/ca (0 when ca is outside [2^-16, 2^16]
+   in magnitude) from the element program. */
 template <class R>
 XT_HD void polar_drift(PState<R> &p, const double length, const double rho, const double ca,
                        const double sa, const double t2, const double ica) {
@@ -152,23 +204,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
     const R n1 = one_plus_delta * xr * sa;
     const R n3 = px * sa;
-#ifdef XTM_PLAIN_DIVISION
-    const R delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
-    const R dz = length - delta_ell / rvv;
-    (void)ica;
-#else
-    const R q2 = xdiv_rcp(xdiv_rcp(n1, R(ca), ica), pz, _pz);
-    const R q4 = xdiv_rcp(xdiv_rcp(n3, R(ca), ica), pz, _pz);
-    const R den = 1 - q4;
-    const R delta_ell = q2 / den;
-    R dz = length - xdiv_rcp(delta_ell, rvv, p.inv_rvv);
-    const bool ok = (ica != 0.0) && num_ok(xval(n1)) && num_ok(xval(n3)) &&
-                    den_ok(xval(pz)) && den_ok(xval(rvv)) && den_ok(xval(den));
-    if (!ok) {
-        const R del = n1 / ca / pz / (1 - n3 / ca / pz);
-        dz = length - del / rvv;
-    }
-#endif
+    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv, p.rvv_ok, length);
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;
@@ -176,13 +212,18 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     p.s = p.s + length;
 }
 
-/* track_magnet_drift.h:91-213 */
 template <class R>
-XT_HD_COLD PState<R> dipole_quad(PState<R> p, const double length, const double k0_,
-                                 const double k1_, const double h) {
-    const R x = p.x, y = p.y, px = p.px, py = p.py, rvv = p.rvv;
-    const R delta_plus_1 = p.delta + 1;
-    const R chi = p.chi;
+struct DQOut { R x, px, y, py, dzeta; };
+
+/* track_magnet_drift.h:91-213.  Out of line with scalar arguments and a
+   five-value result (both passed in VGPRs), so the hot loop keeps neither
+   its registers nor a memory copy of the particle. */
+template <class R>
+XT_HD_COLD DQOut<R> dipole_quad_map(const R x, const R px, const R y, const R py,
+                                    const R delta, const R rvv, const R chi,
+                                    const double length, const double k0_, const double k1_,
+                                    const double h) {
+    const R delta_plus_1 = delta + 1;
     const R k0 = chi * k0_ / delta_plus_1;
     const R k1 = chi * k1_ / delta_plus_1;
     const R Kx = k0 * h + k1;
@@ -250,19 +291,26 @@ XT_HD_COLD PState<R> dipole_quad(PState<R> p, const double length, const double 
         length_ = length_ + 0.5 * (D * D) * length;
     }
     const R dzeta = length - length_ / rvv;
-    p.x = x_;
-    p.px = px_;
-    p.y = y_;
-    p.py = py_;
-    p.zeta = p.zeta + dzeta;
+    return DQOut<R>{x_, px_, y_, py_, dzeta};
+}
+
+template <class R>
+XT_HD void dipole_quad(PState<R> &p, const double length, const double k0, const double k1,
+                       const double h) {
+    const DQOut<R> o = dipole_quad_map(p.x, p.px, p.y, p.py, p.delta, p.rvv, p.chi, length,
+                                       k0, k1, h);
+    p.x = o.x;
+    p.px = o.px;
+    p.y = o.y;
+    p.py = o.py;
+    p.zeta = p.zeta + o.dzeta;
     p.s = p.s + length;
-    return p;
 }
 
 /* ---- multipole kick (track_magnet_kick.h:24-144, 183-228) ----------------- */
 /* literal Horner pass; f = inv-factorial chain f[order], ..., f[0] */
 template <class R>
-XT_HD void kick_pass(PState<R> &p, int64_t order, const double *__restrict__ f,
+XT_HD void kick_pass(R &px, R &py, int64_t order, const double *__restrict__ f,
                      const double *__restrict__ knl, const double *__restrict__ ksl,
                      double factor, double kick_weight, R x, R y, R chi) {
     int64_t index = order;
@@ -280,14 +328,19 @@ XT_HD void kick_pass(PState<R> &p, int64_t order, const double *__restrict__ f,
         dpy_mul = this_ksl * f[j] + zim;
     }
     dpx_mul = -dpx_mul;
-    p.px = p.px + kick_weight * dpx_mul;
-    p.py = p.py + kick_weight * dpy_mul;
+    px = px + kick_weight * dpx_mul;
+    py = py + kick_weight * dpy_mul;
 }
 
 template <class R>
-XT_HD_COLD PState<R> magnet_kick_literal(PState<R> p, double kick_weight,
-                                         const double *__restrict__ d) {
-    const R chi = p.chi, x = p.x, y = p.y;
+struct KickOut { R px, py, zeta; };
+
+/* the unfolded kick of track_magnet_kick.h:24-144; out of line with scalar
+   arguments (see dipole_quad_map) */
+template <class R>
+XT_HD_COLD KickOut<R> magnet_kick_literal(const R x, const R y, R px, R py, R zeta,
+                                          const R delta, const R chi, const R inv_rvv,
+                                          double kick_weight, const double *__restrict__ d) {
     const int64_t order = wbits(d[KD_ORDER]);
     const int64_t order_rel = wbits(d[KD_ORDER_REL]);
     const double length = d[KD_LENGTH];
@@ -298,14 +351,14 @@ XT_HD_COLD PState<R> magnet_kick_literal(PState<R> p, double kick_weight,
     const double *knl_rel = f2 + (order_rel + 1);
     const double *ksl_rel = knl_rel + (order_rel + 1);
     const double *f3 = ksl_rel + (order_rel + 1);
-    kick_pass(p, order, f1, knl, ksl, d[KD_FACTOR], kick_weight, x, y, chi);
-    kick_pass(p, order_rel, f2, knl_rel, ksl_rel, d[KD_FACTOR_REL], kick_weight, x, y, chi);
-    kick_pass(p, 3, f3, d + KD_MAIN, d + KD_MAINS, 1, kick_weight, x, y, chi);
+    kick_pass(px, py, order, f1, knl, ksl, d[KD_FACTOR], kick_weight, x, y, chi);
+    kick_pass(px, py, order_rel, f2, knl_rel, ksl_rel, d[KD_FACTOR_REL], kick_weight, x, y, chi);
+    kick_pass(px, py, 3, f3, d + KD_MAIN, d + KD_MAINS, 1, kick_weight, x, y, chi);
     R dpx = R(0.0), dpy = R(0.0), dzeta = R(0.0);
     if (wbits(d[KD_ROT_FRAME])) {
         double const hl = d[KD_H] * length * kick_weight + d[KD_HXL] * kick_weight;
-        dpx = dpx + hl * (1. + p.delta);
-        R const rv0v = p.inv_rvv;
+        dpx = dpx + hl * (1. + delta);
+        R const rv0v = inv_rvv;
         dzeta = dzeta + -rv0v * hl * x;
     }
     const double htot = d[KD_HTOT];
@@ -314,10 +367,7 @@ XT_HD_COLD PState<R> magnet_kick_literal(PState<R> p, double kick_weight,
     dpx = dpx + -chi * k0c * kick_weight * htot * x;
     dpx = dpx + htot * chi * k1c * kick_weight * (-x * x + 0.5 * y * y);
     dpy = dpy + htot * chi * k1c * kick_weight * x * y;
-    p.px = p.px + dpx;
-    p.py = p.py + dpy;
-    p.zeta = p.zeta + dzeta;
-    return p;
+    return KickOut<R>{px + dpx, py + dpy, zeta + dzeta};
 }
 
 /* Horner pass on precomputed coefficients a[k] = (knl[k]*factor)*f[k],
@@ -345,10 +395,21 @@ XT_HD void magnet_kick(PState<R> &p, const double *__restrict__ a, const double 
     const double w = a[0];
     const double *__restrict__ d = prog + wbits(a[1]);
     const int64_t mask = wbits(a[2]);
-    const bool sane = (xval(p.chi) == 1.0) && xfinite(xval(p.x)) && xfinite(xval(p.y)) &&
-                      xfinite(xval(p.delta)) && xfinite(xval(p.rvv)) && (xval(p.rvv) != 0.0);
+    /* The folded form drops terms that the literal kick computes as an
+       exact zero times a polynomial in x, y and then adds.  That is exact
+       when (i) chi == 1 and delta, rvv are finite with rvv != 0 (fast_ok),
+       (ii) |x|, |y| < 2^511, so x*x and y*y cannot overflow into a NaN
+       through 0*inf, and (iii) px, py, zeta != 0, so adding a zero of
+       either sign leaves them unchanged. */
+    const double ax = xfabs(xval(p.x)), ay = xfabs(xval(p.y));
+    const bool sane = p.fast_ok && ax < 0x1p511 && ay < 0x1p511 && xval(p.px) != 0.0 &&
+                      xval(p.py) != 0.0 && xval(p.zeta) != 0.0;
     if (!sane) {
-        p = magnet_kick_literal(p, w, d);
+        const KickOut<R> o = magnet_kick_literal(p.x, p.y, p.px, p.py, p.zeta, p.delta, p.chi,
+                                                 p.inv_rvv, w, d);
+        p.px = o.px;
+        p.py = o.py;
+        p.zeta = o.zeta;
         return;
     }
     const R x = p.x, y = p.y;
@@ -416,11 +477,15 @@ XT_HD void update_ptau(PState<R> &p, R new_ptau) {
     p.ptau = ptau;
     p.rpp = new_rpp;
     p.inv_rvv = 1. / new_rvv;
+    refresh_flags(p);
 }
 
+/* new ptau after the cavity's energy kick (out of line, scalar arguments) */
 template <class R>
-XT_HD_COLD PState<R> rf_kick(PState<R> p, const double *__restrict__ a, double line_length,
-                        double q0, double t_sim) {
+XT_HD_COLD R rf_kick_ptau(const R zeta, const R ptau_in, const R beta0, const R charge_ratio,
+                          const R chi, const R p0c, const int64_t at_turn,
+                          const double *__restrict__ a, double line_length, double q0,
+                          double t_sim) {
     const double voltage = a[0];
     const double frequency0 = a[1];
     const double harmonic = a[2];
@@ -428,26 +493,23 @@ XT_HD_COLD PState<R> rf_kick(PState<R> p, const double *__restrict__ a, double l
     const double phase = a[4];
     const int64_t absolute_time = wbits(a[5]);
     R phase0 = R(0.0);
-    R const beta0 = p.beta0;
     R frequency = R(frequency0);
     if (harmonic != 0) {
         R const t_rev0 = line_length / (beta0 * C_LIGHT);
         frequency = frequency + (harmonic / t_rev0);
     }
     if (absolute_time == 1) {
-        phase0 = phase0 + 2 * PI * (double)p.at_turn * frequency * t_sim;
+        phase0 = phase0 + 2 * PI * (double)at_turn * frequency * t_sim;
     }
-    R const zeta = p.zeta;
-    R const q = xfabs(q0) * p.charge_ratio;
+    R const q = xfabs(q0) * charge_ratio;
     R const tau = zeta / beta0;
     R const energy_kick =
         q * voltage * xsin(phase0 + DEG2RAD * lag + phase - (2.0 * PI) / C_LIGHT * frequency * tau);
     double const rfmultipole_energy_kick = 0;
-    R ptau = p.ptau;
-    R const mass_ratio = p.charge_ratio / p.chi;
-    ptau = ptau + (energy_kick + rfmultipole_energy_kick) / p.p0c / mass_ratio;
-    update_ptau(p, ptau);
-    return p;
+    R ptau = ptau_in;
+    R const mass_ratio = charge_ratio / chi;
+    ptau = ptau + (energy_kick + rfmultipole_energy_kick) / p0c / mass_ratio;
+    return ptau;
 }
 
 /* ---- element program interpreter ------------------------------------------ */
@@ -476,7 +538,7 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             pc += 7;
             break;
         case OP_DIPQUAD:
-            p = dipole_quad(p, a[0], a[1], a[2], a[3]);
+            dipole_quad(p, a[0], a[1], a[2], a[3]);
             pc += 5;
             break;
         case OP_YOSHIDA4_BEND: {
@@ -531,7 +593,9 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             break;
         }
         case OP_RF_KICK:
-            if (!kill_cavity) p = rf_kick(p, a, line_length, q0, t_sim);
+            if (!kill_cavity)
+                update_ptau(p, rf_kick_ptau(p.zeta, p.ptau, p.beta0, p.charge_ratio, p.chi,
+                                            p.p0c, p.at_turn, a, line_length, q0, t_sim));
             pc += 7;
             break;
         case OP_LIMIT_RECT:
@@ -647,6 +711,7 @@ XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__re
     p.beta0 = P.beta0[i]; p.p0c = P.p0c[i];
     p.at_element = P.at_element[i]; p.at_turn = P.at_turn[i]; p.state = P.state[i];
     p.inv_rvv = 1. / p.rvv;
+    refresh_flags(p);
     const int64_t pid = flag_monitor ? P.particle_id[i] : 0;
     const double q0 = P.q0, t_sim = P.t_sim;
     const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
