@@ -54,9 +54,9 @@ COPY_MIB = 256
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--turns-per-step', type=int, default=2)
+    ap.add_argument('--turns-per-step', type=int, default=1)
     ap.add_argument('--particles', type=int, default=1_000_000)
     ap.add_argument('--cpu-sample-particles', type=int, default=8192)
     ap.add_argument('--cpu-sample-turns', type=int, default=20)
@@ -66,6 +66,11 @@ def parse():
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a plain device copy of known size before tracking '
                          '(used by the traffic child runs)')
+    ap.add_argument('--compile', choices=('auto', 'on', 'off'), default='auto',
+                    help='track with compiled line code (default when possible) or the '
+                         'element-program interpreter')
+    ap.add_argument('--waves-per-eu', type=int, default=0,
+                    help='occupancy target of the compiled kernel (0: library default)')
     ap.add_argument('--rematch', action='store_true')
     return ap.parse_args()
 
@@ -114,7 +119,8 @@ def measure_traffic(args):
                    '--', sys.executable, os.path.abspath(__file__), '--steps', '1',
                    '--warmup', '0', '--turns-per-step', str(args.turns_per_step),
                    '--particles', str(args.particles), '--no-cpu-baseline',
-                   '--traffic', 'none', '--copy-calibration']
+                   '--traffic', 'none', '--copy-calibration', '--compile', args.compile,
+                   '--waves-per-eu', str(args.waves_per_eu)]
             print(f'bench.py: traffic pass {counter}', file=sys.stderr, flush=True)
             with open(os.path.join(tmp, counter + '.log'), 'w') as log:
                 subprocess.run(cmd, stdout=log, stderr=subprocess.STDOUT, timeout=420,
@@ -178,7 +184,10 @@ def main():
         return lattices.collider_ring(k2f=k2f, k2d=k2d)
 
     line = make_line()
-    tracker = line.build_tracker(device='cuda:0')
+    tracker = line.build_tracker(device='cuda:0',
+                                 compile={'auto': 'auto', 'on': True, 'off': False}[args.compile],
+                                 waves_per_eu=args.waves_per_eu)
+    compiled = tracker.backend.is_compiled(tracker._handle)
     ref = line.particle_ref
     tw = optics.twiss(line, ref)
     chroma = optics.chromaticity(line, ref)
@@ -247,6 +256,7 @@ def main():
         'config': {'workload': 'collider ring, 8 arcs x 23 FODO cells, '
                                f'{len(line.elements)} elements, 7 TeV protons',
                    'particles': args.particles, 'turns_per_step': tps,
+                   'kernel': 'compiled line code' if compiled else 'element-program interpreter',
                    'particles_alive_end': n_alive,
                    'tunes': [round(float(tw['qx']), 4), round(float(tw['qy']), 4)],
                    'chromaticity': [round(float(chroma[0]), 3), round(float(chroma[1]), 3)]},

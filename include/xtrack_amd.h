@@ -195,6 +195,34 @@ int32_t xtm_line_destroy(xtm_line *line);
 int64_t xtm_line_num_elements(const xtm_line *line);
 int64_t xtm_line_num_records(const xtm_line *line);
 
+/* Compile line code for a created line and use it for every later
+ * xtm_track_line call.  Counterpart of the reference's per-line kernel
+ * generation and compilation at build_tracker time (xtrack/tracker.py:
+ * 544-751, compiled by xobjects): here the generated source has one
+ * straight-line function per distinct element record with the record's
+ * constants as literals, calling the same element maps as the program
+ * interpreter, and is compiled with hiprtc for gfx950.  waves_per_eu: the
+ * kernel's occupancy target (0: the library default).  Returns
+ * XTM_ERR_UNSUPPORTED, leaving the interpreter in use, when the line has
+ * more distinct records than code is generated for; XTM_ERR_HIP when
+ * compilation or loading fails (message in xtm_last_error). */
+int32_t xtm_line_specialize(xtm_line *line, int32_t waves_per_eu);
+
+/* 1 when the line tracks with generated code, else 0. */
+int32_t xtm_line_is_specialized(const xtm_line *line);
+
+/* Diagnostics (no counterpart in the reference): the source that
+ * xtm_line_specialize compiles, written to buf (capacity cap, NUL
+ * terminated, *len = full length), the record index of each element
+ * (rec_of_elem, num_elements entries, may be NULL), and the element
+ * program the generated code refers to (xtm_line_program_size doubles).
+ * Works on host-only handles; the CPU tests compile the source for the
+ * host. */
+int32_t xtm_line_generate_source(const xtm_line *line, int32_t waves_per_eu, char *buf,
+                                 int64_t cap, int64_t *len, int32_t *rec_of_elem);
+int64_t xtm_line_program_size(const xtm_line *line);
+int32_t xtm_line_copy_program(const xtm_line *line, double *out);
+
 /* Track particles in place, asynchronously on `stream`.  Same arguments and
  * semantics as the reference kernel `track_line` (xtrack/tracker.py:546-564):
  * turns [0, num_turns), elements [ele_start, ele_start+num_ele_track) each
@@ -222,9 +250,11 @@ int32_t xtm_line_count_ops(const xtm_line *line, const double *coords, double p0
 int32_t xtm_synchronize(void *stream);
 
 /* Diagnostics: evaluate the library's device elementary functions
- * (fn: 0 sin, 1 cos, 2 sinh, 3 cosh, 4 exp, 5 tan, 6 sqrt, 7 div by x[i+n])
+ * (fn: 0 sin, 1 cos, 2 sinh, 3 cosh, 4 exp, 5 tan, 6 sqrt, 7 div by x[i+n];
+ * the kernel's in-range sequences: 8 reciprocal, 9 sqrt, 10 div by x[i+n])
  * on n device doubles.  No counterpart in the reference; used to check that
- * the device and host builds of the functions agree bit for bit. */
+ * the device and host builds of the functions agree bit for bit and that the
+ * in-range sequences return the IEEE results. */
 int32_t xtm_math_eval(int32_t fn, const double *x, double *y, int64_t n,
                       void *stream);
 

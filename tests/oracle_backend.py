@@ -120,3 +120,74 @@ class HostKernelBackend(OracleBackend):
             int(track_flags))
         if st != 0:
             raise RuntimeError(f'host kernel status {st}')
+
+
+# ---------------------------------------------------------------------------
+# generated line code compiled for the host (test infrastructure)
+_gen_cache = {}
+
+
+def generated_host_line(types, offsets, params, line_length):
+    """(lib, rec, prog): the library's generated line code for this element
+    table, compiled for the host with g++, plus the record index per element
+    and the element program it refers to."""
+    import hashlib
+    import tempfile
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    _native.check(lib.xtm_line_create(types.ctypes.data, offsets.ctypes.data,
+                                      params.ctypes.data, len(types), line_length, -1,
+                                      ctypes.byref(h)))
+    try:
+        n = ctypes.c_int64()
+        rec = np.zeros(max(len(types), 1), dtype=np.int32)
+        _native.check(lib.xtm_line_generate_source(h, 0, None, 0, ctypes.byref(n),
+                                                   rec.ctypes.data))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _native.check(lib.xtm_line_generate_source(h, 0, buf, n.value + 1, ctypes.byref(n),
+                                                   None))
+        prog = np.zeros(lib.xtm_line_program_size(h), dtype=np.float64)
+        _native.check(lib.xtm_line_copy_program(h, prog.ctypes.data))
+    finally:
+        lib.xtm_line_destroy(h)
+    src = buf.value.decode()
+    key = hashlib.sha1(src.encode()).hexdigest()
+    if key not in _gen_cache:
+        d = tempfile.mkdtemp(prefix='xtm_gen_')
+        cpp = os.path.join(d, 'gen.cpp')
+        so = os.path.join(d, 'gen.so')
+        with open(cpp, 'w') as fh:
+            fh.write(src)
+        subprocess.run(['g++', '-O1', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off',
+                        '-fno-fast-math', f'-I{ROOT}/include',
+                        f'-I{ROOT}/xtrack_amd/csrc', cpp, '-o', so],
+                       check=True, capture_output=True)
+        glib = ctypes.CDLL(so)
+        c = ctypes
+        glib.xtm_line_host.restype = None
+        glib.xtm_line_host.argtypes = [c.c_void_p, c.c_void_p, c.c_double,
+                                       c.POINTER(_native.XtmParticles), c.c_int32, c.c_int32,
+                                       c.c_int32, c.c_int32, c.c_int32, c.c_int32,
+                                       c.POINTER(_native.XtmMonitor), c.c_uint64]
+        _gen_cache[key] = glib
+    return _gen_cache[key], rec, prog, src
+
+
+class GeneratedHostBackend(OracleBackend):
+    """Same protocol, running the library's generated line code on the host."""
+
+    def prepare_line(self, types, offsets, params, line_length):
+        glib, rec, prog, _ = generated_host_line(types, offsets, params, line_length)
+        return glib, rec, prog, line_length
+
+    def track_line(self, handle, particles, num_turns, ele_start, num_ele_track,
+                   flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor,
+                   monitor, track_flags):
+        glib, rec, prog, line_length = handle
+        pst = _native.make_particles_struct(particles)
+        mst = _native.make_monitor_struct(monitor) if monitor is not None else None
+        glib.xtm_line_host(rec.ctypes.data, prog.ctypes.data, line_length, ctypes.byref(pst),
+                           int(num_turns), int(ele_start), int(num_ele_track),
+                           int(flag_end_turn_actions), int(flag_reset_s_at_end_turn),
+                           int(flag_monitor), ctypes.byref(mst) if mst is not None else None,
+                           int(track_flags))

@@ -10,10 +10,21 @@ from oracle_backend import OracleBackend
 
 pytestmark = pytest.mark.gpu
 
+_MODE = {'compile': False}
+
+
+@pytest.fixture(params=[False, True], ids=['interpreter', 'compiled'], autouse=True)
+def _track_mode(request):
+    """Every test runs on the element-program interpreter and on the
+    line's compiled code."""
+    _MODE['compile'] = request.param
+    yield
+
 
 def _pair(line_fn, p):
     line_g = line_fn()
-    line_g.build_tracker(device='cuda:0')
+    line_g.build_tracker(device='cuda:0', compile=_MODE['compile'])
+    assert line_g.tracker.backend.is_compiled(line_g.tracker._handle) == _MODE['compile']
     line_c = line_fn()
     line_c.build_tracker(backend=OracleBackend())
     pg = p.copy(_context='cuda:0')
@@ -102,7 +113,7 @@ def test_global_aperture_after_static_thick_elements():
         return xt.Line(elements=[xt.Drift(length=2), xt.Quadrupole(length=2, k1=0),
                                  xt.Multipole(length=2, isthick=True)])
     line = mk()
-    line.build_tracker(device='cuda:0')
+    line.build_tracker(device='cuda:0', compile=_MODE['compile'])
     for start, expected in ((1, -1), (2, 1), (0, -1)):
         p = xt.Particles(p0c=7e12, px=0.6)
         p.move(_context='cuda:0')
@@ -114,7 +125,7 @@ def test_partial_tracking_matrix():
     """tests/test_tracker.py:152-340: every (start, stop, turns) split."""
     n_elem = 9
     line = xt.Line(elements=[xt.Drift(length=1.) for _ in range(n_elem)])
-    line.build_tracker(device='cuda:0')
+    line.build_tracker(device='cuda:0', compile=_MODE['compile'])
     p0 = xt.Particles(x=[1e-3, -2e-3, 5e-3], y=[2e-3, -4e-3, 3e-3], zeta=1e-2, p0c=7e12)
 
     def run(expected_turn, expected_el, **kw):
@@ -157,7 +168,7 @@ def test_partial_tracking_matrix():
 def test_tbt_monitor_series(ele_start, ele_stop, expected_x):
     """Expected series of xtrack tests/test_tracker.py:577-602."""
     line = xt.Line(elements=[xt.Drift(length=1)] * 5)
-    line.build_tracker(device='cuda:0')
+    line.build_tracker(device='cuda:0', compile=_MODE['compile'])
     p = xt.Particles(px=0.001)
     p.move(_context='cuda:0')
     line.track(p, num_turns=5, turn_by_turn_monitor=True, ele_start=ele_start,

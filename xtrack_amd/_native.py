@@ -76,6 +76,11 @@ def make_monitor_struct(monitor):
     return st
 
 
+XTM_ERR_ARG = -1
+XTM_ERR_UNSUPPORTED = -2
+XTM_ERR_HIP = -3
+
+
 class NativeError(RuntimeError):
     pass
 
@@ -118,6 +123,17 @@ def load():
     lib.xtm_line_count_ops.restype = c.c_int32
     lib.xtm_line_count_ops.argtypes = [c.c_void_p, c.c_void_p, c.c_double, c.c_double,
                                        c.c_double, c.c_void_p]
+    lib.xtm_line_specialize.restype = c.c_int32
+    lib.xtm_line_specialize.argtypes = [c.c_void_p, c.c_int32]
+    lib.xtm_line_is_specialized.restype = c.c_int32
+    lib.xtm_line_is_specialized.argtypes = [c.c_void_p]
+    lib.xtm_line_generate_source.restype = c.c_int32
+    lib.xtm_line_generate_source.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64,
+                                             c.POINTER(c.c_int64), c.c_void_p]
+    lib.xtm_line_program_size.restype = c.c_int64
+    lib.xtm_line_program_size.argtypes = [c.c_void_p]
+    lib.xtm_line_copy_program.restype = c.c_int32
+    lib.xtm_line_copy_program.argtypes = [c.c_void_p, c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

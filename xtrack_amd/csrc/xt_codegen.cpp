@@ -1,0 +1,234 @@
+// xt_codegen.cpp -- line code generation.
+//
+// The element-program interpreter (xt_ops.h: run_element) reads every op
+// word and constant of an element from memory and dispatches on the op at
+// run time.  For a line with few distinct element records (a ring is made
+// of a handful of magnet, drift and aperture types repeated thousands of
+// times) this file writes the same op sequence as straight-line calls of
+// the same element functions, with the constants as exact hexadecimal
+// literals, so the device compiler sees constants instead of loads and no
+// per-op dispatch remains; the per-element dispatch becomes one switch on a
+// 32-bit record index.  Folded multipole-kick passes are unrolled with the
+// zero pattern of their coefficients known.  The arithmetic per particle is
+// unchanged: the generated code calls the functions the interpreter calls,
+// with the values the interpreter would load.
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xtrack_amd.h"
+#include "xt_codegen.h"
+#include "xt_program.h"
+
+namespace {
+
+int64_t wbits(double d) { int64_t u; std::memcpy(&u, &d, 8); return u; }
+
+// exact literal for a double
+std::string lit(double v) {
+    if (std::isnan(v)) return "__builtin_nan(\"\")";
+    if (std::isinf(v)) return v > 0 ? "__builtin_inf()" : "(-__builtin_inf())";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", v);
+    std::string s(buf);
+    if (s[0] == '-') return "(" + s + ")";
+    return s;
+}
+
+std::string lits(const double *a, int n) {
+    std::string s;
+    for (int k = 0; k < n; k++) {
+        s += ", ";
+        s += lit(a[k]);
+    }
+    return s;
+}
+
+const int kNargs[OP_NUM] = XT_OP_NARGS_INIT;
+
+struct Gen {
+    const std::vector<double> &prog;
+    std::string out;
+    std::string err;
+
+    explicit Gen(const std::vector<double> &p) : prog(p) {}
+
+    void line(const std::string &s) { out += "    " + s + "\n"; }
+
+    bool in_range(int64_t pos, int64_t n) const {
+        return pos >= 0 && n >= 0 && pos + n <= (int64_t)prog.size();
+    }
+
+    // one folded Horner pass from the coefficient block at `fp`
+    bool kick_pass(int64_t fp, double w) {
+        if (!in_range(fp, 1)) return false;
+        const int64_t top = wbits(prog[fp]);
+        if (top < 0 || !in_range(fp, 2 * top + 3)) return false;
+        const double *a = &prog[fp + 1];
+        const double *b = &prog[fp + 2 + top];
+        line("    {");
+        line("        double dpx = " + lit(a[top]) + ", dpy = " + lit(b[top]) + ";");
+        for (int64_t k = top - 1; k >= 0; k--) {
+            line("        xt::horner_step(dpx, dpy, x, y, " + lit(a[k]) + ", " + lit(b[k]) + ", " +
+                 (a[k] != 0.0 ? "true" : "false") + ", " + (b[k] != 0.0 ? "true" : "false") +
+                 ");");
+        }
+        line("        xt::kick_pass_apply(p, " + lit(w) + ", dpx, dpy);");
+        line("    }");
+        return true;
+    }
+
+    bool kick(int64_t pc) {
+        const double *a = &prog[pc + 1];
+        const double w = a[0];
+        const int64_t desc = wbits(a[1]);
+        const int64_t mask = wbits(a[2]);
+        if (!in_range(desc, KD_ARR)) return false;
+        line("if (!xt::kick_sane(p)) {");
+        line("    xt::kick_literal_apply(p, " + lit(w) + ", prog + " + std::to_string(desc) + ");");
+        line("} else {");
+        line("    const double x = p.x, y = p.y;");
+        for (int k = 0; k < 3; k++) {
+            if (!(mask & (1 << k))) continue;
+            if (!kick_pass(wbits(prog[desc + KD_FASTOFF + k]), w)) return false;
+        }
+        if (mask & (8 | 16 | 32)) {
+            line(std::string("    xt::kick_curvature(p, x, y, ") + ((mask & 8) ? "true" : "false") +
+                 ", " + ((mask & 16) ? "true" : "false") + ", " + ((mask & 32) ? "true" : "false") +
+                 lits(a + 3, 3) + ");");
+        }
+        line("}");
+        return true;
+    }
+
+    bool record(int64_t pc) {
+        for (int guard = 0; guard < 100000; guard++) {
+            if (!in_range(pc, 1)) return false;
+            const int64_t op = wbits(prog[pc]);
+            if (op < 0 || op >= OP_NUM) return false;
+            if (op == OP_END) return true;
+            const int n = kNargs[op];
+            if (!in_range(pc + 1, n)) return false;
+            const double *a = &prog[pc + 1];
+            const std::string at = "prog + " + std::to_string(pc + 1);
+            switch (op) {
+            case OP_DRIFT_EXP: line("xt::drift_expanded(p" + lits(a, 1) + ");"); break;
+            case OP_DRIFT_EXACT: line("xt::drift_exact(p" + lits(a, 1) + ");"); break;
+            case OP_POLAR: line("xt::polar_drift(p" + lits(a, 6) + ");"); break;
+            case OP_DIPQUAD: line("xt::dipole_quad(p" + lits(a, 4) + ");"); break;
+            case OP_YOSHIDA4_BEND: line("xt::op_yoshida4(p" + lits(a, 14) + ");"); break;
+            case OP_YOSHIDA6_BEND: line("xt::op_yoshida6(p, " + at + ");"); break;
+            case OP_KICK:
+                if (!kick(pc)) return false;
+                break;
+            case OP_EDGE_AXAY: line("xt::op_edge_axay(p, m" + lits(a, 2) + ");"); break;
+            case OP_EDGE_CLEAR_AXAY: line("xt::op_edge_clear_axay(m);"); break;
+            case OP_EDGE_LINEAR: line("xt::op_edge_linear(p" + lits(a, 2) + ");"); break;
+            case OP_RF_KICK:
+                line("xt::op_rf_kick(p, m, " + at +
+                     ", c.line_length, c.q0, c.t_sim, c.kill_cavity());");
+                break;
+            case OP_LIMIT_RECT:
+                line("xt::op_limit_rect(p" + lits(a, 4) + ", c.ignore_local());");
+                break;
+            case OP_LIMIT_ELLIPSE:
+                line("xt::op_limit_ellipse(p" + lits(a, 3) + ", c.ignore_local());");
+                break;
+            case OP_GLOBAL_APER: line("xt::op_global_aper(p, c.ignore_global());"); break;
+            default: return false;
+            }
+            pc += 1 + n;
+        }
+        return false;
+    }
+};
+
+}  // namespace
+
+int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
+                            int32_t waves_per_eu, std::string &src,
+                            std::vector<int32_t> &rec_of_elem, std::string &err) {
+    std::map<int64_t, int32_t> index;
+    for (int64_t e : entry) index.emplace(e, 0);
+    if ((int64_t)index.size() > XT_CODEGEN_MAX_RECORDS) {
+        err = "line has " + std::to_string(index.size()) + " distinct element records (more than " +
+              std::to_string(XT_CODEGEN_MAX_RECORDS) + ")";
+        return XTM_ERR_UNSUPPORTED;
+    }
+    int32_t k = 0;
+    for (auto &kv : index) kv.second = k++;
+    rec_of_elem.resize(entry.size());
+    for (size_t e = 0; e < entry.size(); e++) rec_of_elem[e] = index[entry[e]];
+
+    std::string s;
+    s += "// generated line code: element records as straight-line calls\n";
+    s += "#if defined(__HIPCC__) && !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
+    s += "#include \"xt_ops.h\"\n\n";
+    s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";
+    s += "#else\n#define XT_GEN_FN static inline\n#endif\n\n";
+    s += "namespace xt_gen {\n";
+    for (auto &kv : index) {
+        Gen g(prog);
+        if (!g.record(kv.first)) {
+            err = "cannot generate code for the program at offset " + std::to_string(kv.first);
+            return XTM_ERR_ARG;
+        }
+        s += "XT_GEN_FN void rec_" + std::to_string(kv.second) +
+             "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
+             "    const xt::TrackCtx &c) {\n"
+             "    (void)m; (void)prog; (void)c;\n";
+        s += g.out;
+        s += "}\n\n";
+    }
+    s += "struct Runner {\n"
+         "    const int32_t *__restrict__ rec;\n"
+         "    const double *__restrict__ prog;\n"
+         "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, int64_t e,\n"
+         "                          const xt::TrackCtx &c) const {\n"
+         "        switch (rec[e]) {\n";
+    for (int32_t r = 0; r < k; r++)
+        s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
+             "(p, m, prog, c); break;\n";
+    s += "        default: p.state = -999; break;\n"
+         "        }\n"
+         "    }\n"
+         "};\n"
+         "}  // namespace xt_gen\n\n";
+    s += "#if defined(__HIPCC__)\n";
+    char occ[128];
+    std::snprintf(occ, sizeof occ, "__attribute__((amdgpu_waves_per_eu(%d)))",
+                  waves_per_eu > 0 ? waves_per_eu : 1);
+    s += std::string("extern \"C\" __global__ __launch_bounds__(256) ") +
+         (waves_per_eu > 0 ? occ : "") +
+         "\nvoid xtm_line_kernel(const int32_t *__restrict__ rec, const double *__restrict__ prog,\n"
+         "    double line_length, xtm_particles P, int32_t num_turns, int32_t ele_start,\n"
+         "    int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,\n"
+         "    int32_t flag_monitor, xtm_monitor M, uint64_t flags) {\n"
+         "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
+         "    if (i >= P.capacity) return;\n"
+         "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
+         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, prog}, c, num_turns, ele_start,\n"
+         "                            num_ele_track, flag_end_turn_actions, flag_reset_s,\n"
+         "                            flag_monitor, M);\n"
+         "}\n"
+         "#else\n"
+         "// host build (tests): the same per-particle body over all particles\n"
+         "extern \"C\" void xtm_line_host(const int32_t *rec, const double *prog, double line_length,\n"
+         "    const xtm_particles *P, int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
+         "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
+         "    const xtm_monitor *M, uint64_t flags) {\n"
+         "    xtm_monitor m0 = {};\n"
+         "    const xt::TrackCtx c{flags, line_length, P->q0, P->t_sim};\n"
+         "    for (int64_t i = 0; i < P->capacity; i++)\n"
+         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, prog}, c, num_turns,\n"
+         "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
+         "                                flag_reset_s, flag_monitor, M ? *M : m0);\n"
+         "}\n"
+         "#endif\n";
+    src = std::move(s);
+    return XTM_OK;
+}

@@ -1,0 +1,23 @@
+// xt_codegen.h -- per-line HIP source generated from the element programs.
+#ifndef XT_CODEGEN_H
+#define XT_CODEGEN_H
+#include <cstdint>
+#include <string>
+#include <vector>
+
+// Largest number of distinct element records for which line code is
+// generated; lines with more run on the program interpreter.
+constexpr int64_t XT_CODEGEN_MAX_RECORDS = 512;
+
+// Source of a translation unit that tracks the line: one function per
+// distinct element record with the record's constants as literals, an
+// element runner switching on the record index of each element, and (when
+// compiled as HIP) the kernel `xtm_line_kernel`, whose parameters are those
+// of the interpreter kernel with the entry array replaced by `rec`.  The
+// source includes xt_ops.h.  `rec_of_elem[e]` receives the record index of
+// element e.  `waves_per_eu` (> 0) sets the kernel's occupancy target.
+// Returns an XTM_* status and sets `err` on failure.
+int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
+                            int32_t waves_per_eu, std::string &src,
+                            std::vector<int32_t> &rec_of_elem, std::string &err);
+#endif

@@ -83,17 +83,24 @@ int xt_count_turn(const std::vector<int64_t> &entry, const std::vector<double> &
     const double delta_beta0 = delta * beta0;
     const double ptau_beta0 = std::sqrt(delta_beta0 * delta_beta0 + 2 * delta_beta0 * beta0 + 1) - 1;
     p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.delta = delta;
-    p.ptau = ptau_beta0 / beta0;
     p.rpp = 1 / (1 + delta);
     p.rvv = (1 + delta) / (1 + ptau_beta0);
-    p.s = 0.0; p.ax = 0.0; p.ay = 0.0;
-    p.chi = 1.0; p.charge_ratio = 1.0; p.beta0 = beta0; p.p0c = p0c;
+    p.s = 0.0;
+    p.chi = 1.0;
     p.inv_rvv = FC(1.0) / p.rvv;
-    p.at_element = 0; p.at_turn = 0; p.state = 1;
+    p.at_turn = 0; p.state = 1;
     xt::refresh_flags(p);
+    // fields the maps read from and write to the particle arrays
+    double m_ptau = ptau_beta0 / beta0, m_beta0 = beta0, m_charge_ratio = 1.0, m_p0c = p0c;
+    double m_ax = 0.0, m_ay = 0.0;
+    xtm_particles mp = {};
+    mp.capacity = 1;
+    mp.ptau = &m_ptau; mp.beta0 = &m_beta0; mp.charge_ratio = &m_charge_ratio;
+    mp.p0c = &m_p0c; mp.ax = &m_ax; mp.ay = &m_ay;
+    const xt::PRef m{&mp, 0};
     g = Counts();
     for (size_t e = 0; e < entry.size(); e++) {
-        xt::run_element(p, prog.data(), entry[e], 0, line_length, q0, 0.0);
+        xt::run_element(p, m, prog.data(), entry[e], xt::TrackCtx{0, line_length, q0, 0.0});
         if (p.state <= 0) return XTM_ERR_ARG;   // reference particle must survive
     }
     out[0] = g.add;

@@ -13,14 +13,18 @@
 // Per-particle arithmetic follows the reference expressions operation by
 // operation; the file must be compiled with -ffp-contract=off.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/xtrack_amd.h"
 #include "xt_builder.h"
+#include "xt_codegen.h"
 #include "xt_flops.h"
 #include "xt_math.h"
 #include "xt_program.h"
@@ -36,6 +40,13 @@ constexpr int BLOCK = 256;
    0 leaves the allocation to the compiler. */
 #ifndef XTM_WAVES_PER_EU
 #define XTM_WAVES_PER_EU 4
+#endif
+/* Occupancy target of compiled line code (xtm_line_specialize with
+   waves_per_eu = 0).  Its per-record straight-line code needs fewer
+   registers per op than the interpreter's dispatch loop; on MI355X five
+   waves per SIMD measured fastest on the thick ring (3, 4, 6 slower). */
+#ifndef XTM_COMPILED_WAVES_PER_EU
+#define XTM_COMPILED_WAVES_PER_EU 5
 #endif
 #if XTM_WAVES_PER_EU > 0
 #define XTM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(XTM_WAVES_PER_EU)))
@@ -68,7 +79,10 @@ __global__ void math_kernel(int32_t fn, const double *__restrict__ x, double *__
     case 4: r = xt_exp(v); break;
     case 5: r = xt_tan(v); break;
     case 6: r = XT_SQRT(v); break;
-    default: r = v / x[i + n]; break;
+    case 7: r = v / x[i + n]; break;
+    case 8: r = xt::rcp_ir(v); break;
+    case 9: r = xt::sqrt_ir(v); break;
+    default: r = xt::div_ir(v, x[i + n]); break;
     }
     y[i] = r;
 }
@@ -95,7 +109,77 @@ struct xtm_line {
     double line_length;
     std::vector<int64_t> entry;   // host copies (operation counting)
     std::vector<double> prog;
+    // generated line code (xtm_line_specialize)
+    int32_t *d_rec = nullptr;
+    hipModule_t module = nullptr;
+    hipFunction_t kernel = nullptr;
 };
+
+#ifndef XTM_RTC_DEFINES
+#define XTM_RTC_DEFINES ""
+#endif
+
+extern const int xt_embedded_header_count;
+extern const char *const xt_embedded_header_names[];
+extern const char *const xt_embedded_header_texts[];
+
+namespace {
+
+// code objects of generated line code, by source text (one process may
+// build many lines with identical element records: tests, optics probes)
+std::mutex g_rtc_mutex;
+std::map<std::string, std::vector<char>> g_rtc_cache;
+
+int32_t rtc_compile(const std::string &src, std::vector<char> &code) {
+    {
+        std::lock_guard<std::mutex> lk(g_rtc_mutex);
+        auto it = g_rtc_cache.find(src);
+        if (it != g_rtc_cache.end()) {
+            code = it->second;
+            return XTM_OK;
+        }
+    }
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "xtm_line.hip",
+                                         xt_embedded_header_count, xt_embedded_header_texts,
+                                         xt_embedded_header_names);
+    if (r != HIPRTC_SUCCESS)
+        return fail(XTM_ERR_HIP, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
+    std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                     "-ffp-contract=off", "-fno-fast-math", "-Xclang",
+                                     "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    {
+        const std::string defs = XTM_RTC_DEFINES;
+        size_t a = 0;
+        while (a < defs.size()) {
+            size_t b = defs.find(' ', a);
+            if (b == std::string::npos) b = defs.size();
+            if (b > a) opts.push_back(defs.substr(a, b - a));
+            a = b + 1;
+        }
+    }
+    std::vector<const char *> copts;
+    for (auto &o : opts) copts.push_back(o.c_str());
+    r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    if (r != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return fail(XTM_ERR_HIP, "line code compilation failed: " + log.substr(0, 4000));
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    std::lock_guard<std::mutex> lk(g_rtc_mutex);
+    g_rtc_cache[src] = code;
+    return XTM_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -167,7 +251,81 @@ int32_t xtm_line_destroy(xtm_line *L) {
     if (L->device >= 0) (void)hipSetDevice(L->device);
     if (L->d_entry) (void)hipFree(L->d_entry);
     if (L->d_prog) (void)hipFree(L->d_prog);
+    if (L->d_rec) (void)hipFree(L->d_rec);
+    if (L->module) (void)hipModuleUnload(L->module);
     delete L;
+    return XTM_OK;
+}
+
+int32_t xtm_line_specialize(xtm_line *L, int32_t waves_per_eu) {
+    if (!L) return fail(XTM_ERR_ARG, "xtm_line_specialize: null line");
+    if (L->device < 0)
+        return fail(XTM_ERR_ARG, "xtm_line_specialize: line was created without a device");
+    if (waves_per_eu < 0 || waves_per_eu > 8)
+        return fail(XTM_ERR_ARG, "xtm_line_specialize: waves_per_eu outside 0..8");
+    if (waves_per_eu == 0) waves_per_eu = XTM_COMPILED_WAVES_PER_EU;
+    std::string src, err;
+    std::vector<int32_t> rec;
+    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, src, rec, err);
+    if (st != XTM_OK) return fail(st, "xtm_line_specialize: " + err);
+    std::vector<char> code;
+    st = rtc_compile(src, code);
+    if (st != XTM_OK) return st;
+    hipError_t e = hipSetDevice(L->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipModule_t mod = nullptr;
+    if ((e = hipModuleLoadData(&mod, code.data())) != hipSuccess)
+        return hip_fail(e, "hipModuleLoadData");
+    hipFunction_t fn = nullptr;
+    if ((e = hipModuleGetFunction(&fn, mod, "xtm_line_kernel")) != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return hip_fail(e, "hipModuleGetFunction");
+    }
+    int32_t *d_rec = nullptr;
+    const size_t nr = rec.empty() ? 1 : rec.size();
+    if ((e = hipMalloc(&d_rec, nr * sizeof(int32_t))) != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return hip_fail(e, "hipMalloc");
+    }
+    if (!rec.empty() &&
+        (e = hipMemcpy(d_rec, rec.data(), rec.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(d_rec);
+        (void)hipModuleUnload(mod);
+        return hip_fail(e, "hipMemcpy");
+    }
+    if (L->d_rec) (void)hipFree(L->d_rec);
+    if (L->module) (void)hipModuleUnload(L->module);
+    L->d_rec = d_rec;
+    L->module = mod;
+    L->kernel = fn;
+    return XTM_OK;
+}
+
+int32_t xtm_line_is_specialized(const xtm_line *L) { return (L && L->kernel) ? 1 : 0; }
+
+int32_t xtm_line_generate_source(const xtm_line *L, int32_t waves_per_eu, char *buf,
+                                 int64_t cap, int64_t *len, int32_t *rec_of_elem) {
+    if (!L || !len) return fail(XTM_ERR_ARG, "xtm_line_generate_source: null argument");
+    std::string src, err;
+    std::vector<int32_t> rec;
+    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, src, rec, err);
+    if (st != XTM_OK) return fail(st, "xtm_line_generate_source: " + err);
+    *len = (int64_t)src.size();
+    if (buf && cap > 0) {
+        const int64_t n = (int64_t)src.size() < cap - 1 ? (int64_t)src.size() : cap - 1;
+        std::memcpy(buf, src.data(), (size_t)n);
+        buf[n] = '\0';
+    }
+    if (rec_of_elem) std::memcpy(rec_of_elem, rec.data(), rec.size() * sizeof(int32_t));
+    return XTM_OK;
+}
+
+int64_t xtm_line_program_size(const xtm_line *L) { return L ? (int64_t)L->prog.size() : -1; }
+
+int32_t xtm_line_copy_program(const xtm_line *L, double *out) {
+    if (!L || !out) return fail(XTM_ERR_ARG, "xtm_line_copy_program: null argument");
+    std::memcpy(out, L->prog.data(), L->prog.size() * sizeof(double));
     return XTM_OK;
 }
 
@@ -201,6 +359,22 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
     else std::memset(&M, 0, sizeof(M));
     const int64_t nblocks = (P->capacity + BLOCK - 1) / BLOCK;
     if (nblocks > 0x7fffffff) return fail(XTM_ERR_ARG, "too many particles");
+    if (L->kernel) {
+        xtm_particles Pv = *P;
+        const int32_t *rec = L->d_rec;
+        const double *prog = L->d_prog;
+        double line_length = L->line_length;
+        int32_t a_turns = num_turns, a_start = ele_start, a_n = num_ele_track;
+        int32_t a_end = flag_end_turn_actions, a_reset = flag_reset_s_at_end_turn;
+        int32_t a_mon = flag_monitor;
+        uint64_t a_flags = track_flags;
+        void *args[] = {&rec, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
+                        &a_end, &a_reset, &a_mon, &M, &a_flags};
+        e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
+                                  (hipStream_t)stream, args, nullptr);
+        if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
+        return XTM_OK;
+    }
     hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
                        (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length, *P,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
@@ -225,7 +399,7 @@ int32_t xtm_synchronize(void *stream) {
 }
 
 int32_t xtm_math_eval(int32_t fn, const double *x, double *y, int64_t n, void *stream) {
-    if (fn < 0 || fn > 7 || n < 0 || (n > 0 && (!x || !y)))
+    if (fn < 0 || fn > 10 || n < 0 || (n > 0 && (!x || !y)))
         return fail(XTM_ERR_ARG, "xtm_math_eval: invalid arguments");
     if (n == 0) return XTM_OK;
     const int64_t nb = (n + BLOCK - 1) / BLOCK;

@@ -16,15 +16,26 @@
  * term).  A term whose coefficient is exactly +-0 contributes
  * (+-0) * v = +-0 to a sum when v is finite, which leaves the sum unchanged
  * except possibly for the sign of a zero result.  The fast path therefore
- * skips such terms only for a lane whose operands are all finite and whose
- * chi is exactly 1 (so chi*k == k); any other lane runs the literal code.
- * Results are identical to the literal evaluation as IEEE values (+0 and -0
- * compare equal).
+ * skips such terms only for a lane where every such v is finite (chi == 1,
+ * finite delta, rvv, 1/rvv and |x|, |y| < 2^511) and where px, py and zeta
+ * are non-zero, so that a zero of either sign added to them changes no bit;
+ * any other lane runs the literal code.  Results are bit-identical to the
+ * literal evaluation, the sign of zero included.
+ *
+ * Division and square root.  Every quotient and square root is the
+ * correctly rounded IEEE result, as in the reference.  Where operands are
+ * inside ranges that rule out over- and underflow, the device computes
+ * them with shorter exact sequences (reciprocal multiplication with
+ * Markstein's correction, and the hardware estimates refined by Newton
+ * steps); a lane outside the ranges recomputes the map with IEEE
+ * operations.
  */
 #ifndef XT_OPS_H
 #define XT_OPS_H
 
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "xt_math.h"
 #include "xt_program.h"
@@ -48,19 +59,31 @@ constexpr double GLOBAL_XY_LIMIT = 1.0;   // default line config, xtrack/line.py
 /* flag bits of OP_KICK's mask argument (which folded terms are present) */
 enum { KF_P1 = 1, KF_P2 = 2, KF_P3 = 4, KF_HL = 8, KF_G0 = 16, KF_H1 = 32 };
 
+/* Particle state held in registers through a launch.  Fields that the
+   element maps use rarely -- ptau (cavity only), ax/ay (dipole edges only),
+   charge_ratio, beta0, p0c (cavity only) -- stay in the particle arrays
+   and are read and written there (PRef); at_element is not held at all:
+   every element passed adds one, so it is a per-particle base plus the
+   element index (track_particle). */
 template <class R>
 struct PState {
-    R x, px, y, py, zeta, delta, ptau, rpp, rvv, s, ax, ay;
-    R chi, charge_ratio, beta0, p0c;
+    R x, px, y, py, zeta, delta, rpp, rvv, s;
+    R chi;
     /* 1./rvv, kept equal to that IEEE quotient: set when the particle is
        loaded and whenever rvv changes (update_ptau) */
     R inv_rvv;
-    int64_t at_element, at_turn, state;
+    int64_t at_turn, state;
     /* per-particle predicates of quantities that change only at load and in
        update_ptau (refresh_flags): the folded multipole kick may be used
        (chi == 1, finite delta, rvv and 1/rvv, rvv != 0); rvv is in the
        range of the reciprocal division (den_ok) */
     bool fast_ok, rvv_ok;
+};
+
+/* the particle's slot in the arrays, for the fields not held in PState */
+struct PRef {
+    const xtm_particles *P;
+    int64_t i;
 };
 
 XT_HD int64_t wbits(double d) { int64_t u; __builtin_memcpy(&u, &d, 8); return u; }
@@ -100,6 +123,51 @@ XT_HD bool den_ok(double v) {
     const double a = XT_FABS(v);
     return a >= 0x1p-16 && a <= 0x1p16;
 }
+
+/* ---- in-range IEEE reciprocal, quotient and square root (device) --------
+   The general IEEE sequences for 1/b, a/b and sqrt(x) on CDNA are: scale
+   the operands away from over/underflow (v_div_scale / ldexp), refine the
+   hardware estimate (v_rcp_f64 / v_rsq_f64) by Newton steps, apply one
+   remainder correction, and patch special values (v_div_fixup / class
+   test).  For operands in the ranges below the scaling and the patching are
+   identities, so the functions here run only the refinement and the
+   correction -- the same operations on the same values, hence the same
+   correctly rounded results, in 7-10 instead of 11-17 instructions.
+   Ranges (checked by the caller): rcp_ir: |b| in [2^-16, 2^16];
+   div_ir: that and |a| in [2^-940, 2^940]; sqrt_ir: x in [2^-32, 2^32]. */
+#if defined(__HIPCC__)
+__device__ __forceinline__ double rcp_ir(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = XT_FMA(-b, y, 1.0);
+    y = XT_FMA(y, e, y);
+    e = XT_FMA(-b, y, 1.0);
+    y = XT_FMA(y, e, y);
+    const double r = XT_FMA(-b, y, 1.0);
+    return XT_FMA(r, y, y);
+}
+__device__ __forceinline__ double div_ir(double a, double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = XT_FMA(-b, y, 1.0);
+    y = XT_FMA(y, e, y);
+    e = XT_FMA(-b, y, 1.0);
+    y = XT_FMA(y, e, y);
+    const double q = a * y;
+    const double r = XT_FMA(-b, q, a);
+    return XT_FMA(r, y, q);
+}
+__device__ __forceinline__ double sqrt_ir(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = XT_FMA(-h, g, 0.5);
+    g = XT_FMA(g, r, g);
+    h = XT_FMA(h, r, h);
+    double d = XT_FMA(-g, g, x);
+    g = XT_FMA(d, h, g);
+    d = XT_FMA(-g, g, x);
+    return XT_FMA(d, h, g);
+}
+#endif
 
 template <class R>
 XT_HD void refresh_flags(PState<R> &p) {
@@ -182,12 +250,94 @@ XT_HD R polar_dz(R n1, R n3, double ca, double ica, R pz, R _pz, R rvv, R inv_rv
     return length - delta_ell / rvv;
 }
 
+struct PolarOut { double x, px, y, dz; };
+
+/* track_magnet_drift.h:45-87 with IEEE operations throughout: the fallback
+   of polar_drift_ir, out of line (scalar arguments, four results in VGPRs). */
+XT_HD_COLD PolarOut polar_plain(const double x, const double px, const double y,
+                                const double py, const double one_plus_delta,
+                                const double rvv, const double length, const double rho,
+                                const double ca, const double sa, const double t2) {
+    const double pz = XT_SQRT(one_plus_delta * one_plus_delta - px * px - py * py);
+    const double _pz = 1 / pz;
+    const double pxt = px * _pz;
+    const double sapxt = sa * pxt;
+    const double _ptt = 1 / (ca - sapxt);
+    const double xr = x + rho;
+    const double pst = xr * sa * _pz * _ptt;
+    PolarOut o;
+    o.x = (x + rho * (t2 + sapxt)) * _ptt;
+    o.px = ca * px + sa * pz;
+    o.y = y + pst * py;
+    const double delta_ell = one_plus_delta * xr * sa / ca / pz / (1 - px * sa / ca / pz);
+    o.dz = length - delta_ell / rvv;
+    return o;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* The polar drift with in-range arithmetic: sqrt_ir for pz, rcp_ir for 1/pz
+   and 1/(ca - sa px/pz), div_rcp with the reciprocals ica = RN(1/ca),
+   1/pz and inv_rvv for the five quotients by ca, pz and rvv, div_ir for the
+   division by 1 - n3/ca/pz.  Range argument: with the checked ranges
+   (radicand in [2^-32, 2^32], so pz in [2^-16, 2^16]; ca, ca - sa px/pz,
+   1 - n3/ca/pz and rvv in [2^-16, 2^16] in magnitude; |n1|, |n3| in
+   [2^-900, 2^900]) every dividend lies in [2^-948, 2^948].  Lanes outside
+   (including px == 0, which makes n3 zero) take polar_plain. */
+__device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double length,
+                                               const double rho, const double ca,
+                                               const double sa, const double t2,
+                                               const double ica) {
+    const double rvv = p.rvv;
+    const double x = p.x, y = p.y, px = p.px, py = p.py;
+    const double one_plus_delta = p.delta + 1.0;
+    const double rad = one_plus_delta * one_plus_delta - px * px - py * py;
+    const double pz = sqrt_ir(rad);
+    const double _pz = rcp_ir(pz);
+    const double pxt = px * _pz;
+    const double sapxt = sa * pxt;
+    const double tden = ca - sapxt;
+    const double _ptt = rcp_ir(tden);
+    const double xr = x + rho;
+    const double pst = xr * sa * _pz * _ptt;
+    double new_x = (x + rho * (t2 + sapxt)) * _ptt;
+    double new_px = ca * px + sa * pz;
+    double new_y = y + pst * py;
+    const double n1 = one_plus_delta * xr * sa;
+    const double n3 = px * sa;
+    const double q2 = div_rcp(div_rcp(n1, ca, ica), pz, _pz);
+    const double q4 = div_rcp(div_rcp(n3, ca, ica), pz, _pz);
+    const double den = 1 - q4;
+    const double delta_ell = div_ir(q2, den);
+    double dz = length - div_rcp(delta_ell, rvv, p.inv_rvv);
+    const bool ok = p.rvv_ok && (ica != 0.0) && rad >= 0x1p-32 && rad <= 0x1p32 &&
+                    den_ok(tden) && num_ok(n1) && num_ok(n3) && den_ok(den);
+    if (!ok) {
+        const PolarOut o = polar_plain(x, px, y, py, one_plus_delta, rvv, length, rho, ca, sa, t2);
+        new_x = o.x;
+        new_px = o.px;
+        new_y = o.y;
+        dz = o.dz;
+    }
+    p.x = new_x;
+    p.px = new_px;
+    p.y = new_y;
+    p.zeta = p.zeta + dz;
+    p.s = p.s + length;
+}
+#endif
+
 /* track_magnet_drift.h:45-87 with rho = 1/h, ca = cos(h s), sa = sin(h s),
    t2 = 2 sin(h s / 2)^2 and ica = 1/ca (0 when ca is outside [2^-16, 2^16]
    in magnitude) from the element program. */
 template <class R>
 XT_HD void polar_drift(PState<R> &p, const double length, const double rho, const double ca,
                        const double sa, const double t2, const double ica) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(XTM_PLAIN_DIVISION)
+    if constexpr (std::is_same<R, double>::value) {
+        polar_drift_ir(p, length, rho, ca, sa, t2, ica);
+        return;
+    }
+#endif
     const R rvv = p.rvv;
     const R x = p.x, y = p.y, px = p.px, py = p.py;
     const R one_plus_delta = p.delta + 1.0;
@@ -370,111 +520,125 @@ XT_HD_COLD KickOut<R> magnet_kick_literal(const R x, const R y, R px, R py, R ze
     return KickOut<R>{px + dpx, py + dpy, zeta + dzeta};
 }
 
-/* Horner pass on precomputed coefficients a[k] = (knl[k]*factor)*f[k],
-   b[k] likewise (valid for chi == 1), starting at the highest non-zero
-   index `top` (> = 0); zero coefficients skip their addition. */
+/* ---- folded multipole kick, in steps shared by the element-program
+   interpreter and by generated line code ----------------------------------- */
+
+/* The folded form drops terms that the literal kick computes as an exact
+   zero times a polynomial in x, y and then adds.  That is exact when
+   (i) chi == 1 and delta, rvv, 1/rvv are finite with rvv != 0 (fast_ok),
+   (ii) |x|, |y| < 2^511, so x*x and y*y cannot overflow into a NaN through
+   0*inf, and (iii) px, py, zeta != 0, so adding a zero of either sign
+   leaves them unchanged. */
 template <class R>
-XT_HD void kick_pass_fast(PState<R> &p, int64_t top, const double *__restrict__ a,
-                          const double *__restrict__ b, double kick_weight, R x, R y) {
-    R dpx_mul = R(a[top]);
-    R dpy_mul = R(b[top]);
-    for (int64_t k = top - 1; k >= 0; k--) {
-        R const zre = dpx_mul * x - dpy_mul * y;
-        R const zim = dpx_mul * y + dpy_mul * x;
-        dpx_mul = (a[k] != 0.0) ? a[k] + zre : zre;
-        dpy_mul = (b[k] != 0.0) ? b[k] + zim : zim;
-    }
-    p.px = p.px + kick_weight * (-dpx_mul);
-    p.py = p.py + kick_weight * dpy_mul;
+XT_HD bool kick_sane(const PState<R> &p) {
+    const double ax = xfabs(xval(p.x)), ay = xfabs(xval(p.y));
+    return p.fast_ok && ax < 0x1p511 && ay < 0x1p511 && xval(p.px) != 0.0 &&
+           xval(p.py) != 0.0 && xval(p.zeta) != 0.0;
 }
 
-/* OP_KICK args: w desc mask hl g0 h1 (top1 top2 top3 as int bits in desc
-   extension, see builder) */
+/* the literal kick for lanes that fail kick_sane; d = kick descriptor */
+template <class R>
+XT_HD void kick_literal_apply(PState<R> &p, double w, const double *__restrict__ d) {
+    const KickOut<R> o = magnet_kick_literal(p.x, p.y, p.px, p.py, p.zeta, p.delta, p.chi,
+                                             p.inv_rvv, w, d);
+    p.px = o.px;
+    p.py = o.py;
+    p.zeta = o.zeta;
+}
+
+/* One Horner step on the precomputed coefficients a = (knl[k]*factor)*f[k]
+   and b likewise (chi == 1); a zero coefficient skips its addition. */
+template <class R>
+XT_HD void horner_step(R &dpx, R &dpy, const R x, const R y, const double a, const double b,
+                       const bool has_a, const bool has_b) {
+    R const zre = dpx * x - dpy * y;
+    R const zim = dpx * y + dpy * x;
+    dpx = has_a ? a + zre : zre;
+    dpy = has_b ? b + zim : zim;
+}
+
+template <class R>
+XT_HD void kick_pass_apply(PState<R> &p, const double w, const R dpx, const R dpy) {
+    p.px = p.px + w * (-dpx);
+    p.py = p.py + w * dpy;
+}
+
+/* curvature terms of track_magnet_kick.h:120-140 for chi == 1 with the
+   element-uniform factors hl, g0 = ((-k0c)*w)*htot, h1 = (htot*k1c)*w */
+template <class R>
+XT_HD void kick_curvature(PState<R> &p, const R x, const R y, const bool has_hl,
+                          const bool has_g0, const bool has_h1, const double hl,
+                          const double g0, const double h1) {
+    R dpx = R(0.0), dpy = R(0.0), dzeta = R(0.0);
+    bool hx = false;
+    if (has_hl) {
+        dpx = hl * (1. + p.delta);
+        hx = true;
+        R const rv0v = p.inv_rvv;
+        dzeta = -rv0v * hl * x;
+    }
+    if (has_g0) {
+        /* -chi*k0c*w*htot*x with chi == 1: (((-k0c)*w)*htot) * x */
+        const R t = g0 * x;
+        dpx = hx ? dpx + t : t;
+        hx = true;
+    }
+    if (has_h1) {
+        /* htot*chi*k1c*w*(...) with chi == 1: ((htot*k1c)*w) */
+        const R t = h1 * (-x * x + 0.5 * y * y);
+        dpx = hx ? dpx + t : t;
+        hx = true;
+        dpy = h1 * x * y;
+    }
+    if (hx) p.px = p.px + dpx;
+    if (has_h1) p.py = p.py + dpy;
+    if (has_hl) p.zeta = p.zeta + dzeta;
+}
+
+/* Horner pass on a coefficient block [top, a[0..top], b[0..top]] */
+template <class R>
+XT_HD void kick_pass_fast(PState<R> &p, const double *__restrict__ fp, double kick_weight,
+                          R x, R y) {
+    const int64_t top = wbits(fp[0]);
+    const double *a = fp + 1, *b = fp + 2 + top;
+    R dpx_mul = R(a[top]);
+    R dpy_mul = R(b[top]);
+    for (int64_t k = top - 1; k >= 0; k--)
+        horner_step(dpx_mul, dpy_mul, x, y, a[k], b[k], a[k] != 0.0, b[k] != 0.0);
+    kick_pass_apply(p, kick_weight, dpx_mul, dpy_mul);
+}
+
+/* OP_KICK args: w desc mask hl g0 h1; the descriptor holds the literal
+   parameters and the offsets of the three coefficient blocks */
 template <class R>
 XT_HD void magnet_kick(PState<R> &p, const double *__restrict__ a, const double *__restrict__ prog) {
     const double w = a[0];
     const double *__restrict__ d = prog + wbits(a[1]);
     const int64_t mask = wbits(a[2]);
-    /* The folded form drops terms that the literal kick computes as an
-       exact zero times a polynomial in x, y and then adds.  That is exact
-       when (i) chi == 1 and delta, rvv are finite with rvv != 0 (fast_ok),
-       (ii) |x|, |y| < 2^511, so x*x and y*y cannot overflow into a NaN
-       through 0*inf, and (iii) px, py, zeta != 0, so adding a zero of
-       either sign leaves them unchanged. */
-    const double ax = xfabs(xval(p.x)), ay = xfabs(xval(p.y));
-    const bool sane = p.fast_ok && ax < 0x1p511 && ay < 0x1p511 && xval(p.px) != 0.0 &&
-                      xval(p.py) != 0.0 && xval(p.zeta) != 0.0;
-    if (!sane) {
-        const KickOut<R> o = magnet_kick_literal(p.x, p.y, p.px, p.py, p.zeta, p.delta, p.chi,
-                                                 p.inv_rvv, w, d);
-        p.px = o.px;
-        p.py = o.py;
-        p.zeta = o.zeta;
+    if (!kick_sane(p)) {
+        kick_literal_apply(p, w, d);
         return;
     }
     const R x = p.x, y = p.y;
-    /* fast coefficients follow the literal descriptor: per pass
-       [top, a[0..n], b[0..n]] starting at d[KD_FAST] offset stored in d */
-    const double *fp = prog + wbits(d[KD_FASTOFF]);
-    if (mask & KF_P1) {
-        const int64_t top = wbits(fp[0]);
-        kick_pass_fast(p, top, fp + 1, fp + 2 + top, w, x, y);
-    }
-    fp = prog + wbits(d[KD_FASTOFF + 1]);
-    if (mask & KF_P2) {
-        const int64_t top = wbits(fp[0]);
-        kick_pass_fast(p, top, fp + 1, fp + 2 + top, w, x, y);
-    }
-    fp = prog + wbits(d[KD_FASTOFF + 2]);
-    if (mask & KF_P3) {
-        const int64_t top = wbits(fp[0]);
-        kick_pass_fast(p, top, fp + 1, fp + 2 + top, w, x, y);
-    }
-    if (mask & (KF_HL | KF_G0 | KF_H1)) {
-        R dpx, dpy;
-        bool hx = false, hy = false;
-        R dzeta;
-        bool hz = false;
-        if (mask & KF_HL) {
-            const double hl = a[3];
-            dpx = hl * (1. + p.delta);
-            hx = true;
-            R const rv0v = p.inv_rvv;
-            dzeta = -rv0v * hl * x;
-            hz = true;
-        }
-        if (mask & KF_G0) {
-            /* -chi*k0c*w*htot*x with chi == 1: (((-k0c)*w)*htot) * x */
-            const R t = a[4] * x;
-            dpx = hx ? dpx + t : t;
-            hx = true;
-        }
-        if (mask & KF_H1) {
-            /* htot*chi*k1c*w*(...) with chi == 1: ((htot*k1c)*w) */
-            const double h1 = a[5];
-            const R t = h1 * (-x * x + 0.5 * y * y);
-            dpx = hx ? dpx + t : t;
-            hx = true;
-            dpy = h1 * x * y;
-            hy = true;
-        }
-        if (hx) p.px = p.px + dpx;
-        if (hy) p.py = p.py + dpy;
-        if (hz) p.zeta = p.zeta + dzeta;
-    }
+    if (mask & KF_P1) kick_pass_fast(p, prog + wbits(d[KD_FASTOFF]), w, x, y);
+    if (mask & KF_P2) kick_pass_fast(p, prog + wbits(d[KD_FASTOFF + 1]), w, x, y);
+    if (mask & KF_P3) kick_pass_fast(p, prog + wbits(d[KD_FASTOFF + 2]), w, x, y);
+    if (mask & (KF_HL | KF_G0 | KF_H1))
+        kick_curvature(p, x, y, (mask & KF_HL) != 0, (mask & KF_G0) != 0, (mask & KF_H1) != 0,
+                       a[3], a[4], a[5]);
 }
 
 /* ---- RF (track_rf.h:18-167) and energy update (custom_api.h:21-32, 196-216) */
 template <class R>
-XT_HD void update_ptau(PState<R> &p, R new_ptau) {
-    R const beta0 = p.beta0;
+XT_HD void update_ptau(PState<R> &p, R new_ptau, const PRef &m) {
+    R const beta0 = R(m.P->beta0[m.i]);
     R const ptau = new_ptau;
     R const irpp = xsqrt(ptau * ptau + 2 * ptau / beta0 + 1);
     R const new_rpp = 1. / irpp;
     R const new_rvv = irpp / (1 + beta0 * ptau);
     p.delta = irpp - 1;
     p.rvv = new_rvv;
-    p.ptau = ptau;
+    m.P->ptau[m.i] = xval(ptau);
     p.rpp = new_rpp;
     p.inv_rvv = 1. / new_rvv;
     refresh_flags(p);
@@ -512,13 +676,113 @@ XT_HD_COLD R rf_kick_ptau(const R zeta, const R ptau_in, const R beta0, const R 
     return ptau;
 }
 
+/* ---- element ops, shared by the interpreter and generated line code ------- */
+
+/* track_magnet_drift.h:500-509: four polar drifts with the k0 kicks of the
+   fourth-order Yoshida step; sub-steps a and b carry their own constants */
+template <class R>
+XT_HD void op_yoshida4(PState<R> &p, const double L, const double rho, const double c1,
+                       const double c2, const double la, const double caa, const double saa,
+                       const double t2a, const double ia, const double lb, const double cab,
+                       const double sab, const double t2b, const double ib) {
+    polar_drift(p, la, rho, caa, saa, t2a, ia);
+    p.px = p.px - c1 * p.chi * L;
+    polar_drift(p, lb, rho, cab, sab, t2b, ib);
+    p.px = p.px - c2 * p.chi * L;
+    polar_drift(p, lb, rho, cab, sab, t2b, ib);
+    p.px = p.px - c1 * p.chi * L;
+    polar_drift(p, la, rho, caa, saa, t2a, ia);
+}
+
+/* track_magnet_drift.h:510-549 (a = the 26 op arguments) */
+template <class R>
+XT_HD void op_yoshida6(PState<R> &p, const double *__restrict__ a) {
+    const double L = a[0], rho = a[1];
+    const double *kc = a + 2;
+    const double *dd = a + 6;
+    for (int j = 0; j < 8; j++) {
+        const int di = j < 4 ? j : 7 - j;
+        const double *s = dd + 5 * di;
+        polar_drift(p, s[0], rho, s[1], s[2], s[3], s[4]);
+        if (j < 7) {
+            const int ki = j < 4 ? j : 6 - j;
+            p.px = p.px - kc[ki] * p.chi * L;
+        }
+    }
+}
+
+template <class R>
+XT_HD void op_edge_axay(const PState<R> &p, const PRef &m, const double ca, const double cb) {
+    m.P->ax[m.i] = xval(ca * (p.y - 0.0));
+    m.P->ay[m.i] = xval(cb * (p.x - 0.0));
+}
+
+XT_HD void op_edge_clear_axay(const PRef &m) {
+    m.P->ax[m.i] = 0.;
+    m.P->ay[m.i] = 0.;
+}
+
+template <class R>
+XT_HD void op_edge_linear(PState<R> &p, const double r21, const double r43) {
+    const R x = p.x, y = p.y, chi = p.chi;
+    p.px = p.px + chi * r21 * x;
+    p.py = p.py + chi * r43 * y;
+}
+
+/* a = the six RF arguments */
+template <class R>
+XT_HD void op_rf_kick(PState<R> &p, const PRef &m, const double *__restrict__ a,
+                      const double line_length, const double q0, const double t_sim,
+                      const bool kill_cavity) {
+    if (kill_cavity) return;
+    update_ptau(p,
+                rf_kick_ptau(p.zeta, R(m.P->ptau[m.i]), R(m.P->beta0[m.i]),
+                             R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), p.at_turn, a,
+                             line_length, q0, t_sim),
+                m);
+}
+
+/* limitrect.h:10-38 */
+template <class R>
+XT_HD void op_limit_rect(PState<R> &p, const double min_x, const double max_x,
+                         const double min_y, const double max_y, const bool ignore_local) {
+    if (ignore_local) return;
+    const double x = xval(p.x), y = xval(p.y);
+    const bool is_alive = (x >= min_x) && (x <= max_x) && (y >= min_y) && (y <= max_y);
+    if (!is_alive) p.state = 0;
+}
+
+/* limitellipse.h:13-38 */
+template <class R>
+XT_HD void op_limit_ellipse(PState<R> &p, const double a_squ, const double b_squ,
+                            const double a_b_squ, const bool ignore_local) {
+    if (ignore_local) return;
+    const R temp = p.x * p.x * b_squ + p.y * p.y * a_squ;
+    if (!(xval(temp) <= a_b_squ)) p.state = 0;
+}
+
+/* local_particle_custom_api.h:259-287 */
+template <class R>
+XT_HD void op_global_aper(PState<R> &p, const bool ignore_global) {
+    if (ignore_global) return;
+    const double x = xval(p.x), y = xval(p.y);
+    const bool within = (x >= -GLOBAL_XY_LIMIT) && (x <= GLOBAL_XY_LIMIT) &&
+                        (y >= -GLOBAL_XY_LIMIT) && (y <= GLOBAL_XY_LIMIT);
+    if (p.state > 0 && !within) p.state = -1;
+}
+
+struct TrackCtx {
+    uint64_t flags;
+    double line_length, q0, t_sim;
+    XT_HD bool ignore_local() const { return (flags >> 4) & 1; }   // XS_FLAG_IGNORE_LOCAL_APERTURE
+    XT_HD bool ignore_global() const { return (flags >> 3) & 1; }  // XS_FLAG_IGNORE_GLOBAL_APERTURE
+    XT_HD bool kill_cavity() const { return (flags >> 2) & 1; }    // XS_FLAG_KILL_CAVITY_KICK
+};
+
 /* ---- element program interpreter ------------------------------------------ */
 template <class R>
-XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc, uint64_t flags,
-                       double line_length, double q0, double t_sim) {
-    const bool ignore_local = (flags >> 4) & 1;   // XS_FLAG_IGNORE_LOCAL_APERTURE
-    const bool ignore_global = (flags >> 3) & 1;  // XS_FLAG_IGNORE_GLOBAL_APERTURE
-    const bool kill_cavity = (flags >> 2) & 1;    // XS_FLAG_KILL_CAVITY_KICK
+XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ prog, int64_t pc,
+                       const TrackCtx &c) {
     for (;;) {
         const int op = (int)wbits(prog[pc]);
         const double *__restrict__ a = prog + pc + 1;
@@ -541,85 +805,45 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
             dipole_quad(p, a[0], a[1], a[2], a[3]);
             pc += 5;
             break;
-        case OP_YOSHIDA4_BEND: {
-            /* track_magnet_drift.h:500-509 */
-            const double L = a[0], rho = a[1], c1 = a[2], c2 = a[3];
-            polar_drift(p, a[4], rho, a[5], a[6], a[7], a[8]);
-            p.px = p.px - c1 * p.chi * L;
-            polar_drift(p, a[9], rho, a[10], a[11], a[12], a[13]);
-            p.px = p.px - c2 * p.chi * L;
-            polar_drift(p, a[9], rho, a[10], a[11], a[12], a[13]);
-            p.px = p.px - c1 * p.chi * L;
-            polar_drift(p, a[4], rho, a[5], a[6], a[7], a[8]);
+        case OP_YOSHIDA4_BEND:
+            op_yoshida4(p, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10],
+                        a[11], a[12], a[13]);
             pc += 15;
             break;
-        }
-        case OP_YOSHIDA6_BEND: {
-            /* track_magnet_drift.h:510-549 */
-            const double L = a[0], rho = a[1];
-            const double *kc = a + 2;
-            const double *dd = a + 6;
-            for (int j = 0; j < 8; j++) {
-                const int di = j < 4 ? j : 7 - j;
-                const double *s = dd + 5 * di;
-                polar_drift(p, s[0], rho, s[1], s[2], s[3], s[4]);
-                if (j < 7) {
-                    const int ki = j < 4 ? j : 6 - j;
-                    p.px = p.px - kc[ki] * p.chi * L;
-                }
-            }
+        case OP_YOSHIDA6_BEND:
+            op_yoshida6(p, a);
             pc += 27;
             break;
-        }
         case OP_KICK:
             magnet_kick(p, a, prog);
             pc += 1 + XT_NARGS_KICK;
             break;
         case OP_EDGE_AXAY:
-            p.ax = a[0] * (p.y - 0.0);
-            p.ay = a[1] * (p.x - 0.0);
+            op_edge_axay(p, m, a[0], a[1]);
             pc += 3;
             break;
         case OP_EDGE_CLEAR_AXAY:
-            p.ax = R(0.);
-            p.ay = R(0.);
+            op_edge_clear_axay(m);
             pc += 1;
             break;
-        case OP_EDGE_LINEAR: {
-            const R x = p.x, y = p.y, chi = p.chi;
-            p.px = p.px + chi * a[0] * x;
-            p.py = p.py + chi * a[1] * y;
+        case OP_EDGE_LINEAR:
+            op_edge_linear(p, a[0], a[1]);
             pc += 3;
             break;
-        }
         case OP_RF_KICK:
-            if (!kill_cavity)
-                update_ptau(p, rf_kick_ptau(p.zeta, p.ptau, p.beta0, p.charge_ratio, p.chi,
-                                            p.p0c, p.at_turn, a, line_length, q0, t_sim));
+            op_rf_kick(p, m, a, c.line_length, c.q0, c.t_sim, c.kill_cavity());
             pc += 7;
             break;
         case OP_LIMIT_RECT:
-            if (!ignore_local) {
-                const double x = xval(p.x), y = xval(p.y);
-                const bool is_alive = (x >= a[0]) && (x <= a[1]) && (y >= a[2]) && (y <= a[3]);
-                if (!is_alive) p.state = 0;
-            }
+            op_limit_rect(p, a[0], a[1], a[2], a[3], c.ignore_local());
             pc += 5;
             break;
         case OP_LIMIT_ELLIPSE:
-            if (!ignore_local) {
-                const R temp = p.x * p.x * a[1] + p.y * p.y * a[0];
-                if (!(xval(temp) <= a[2])) p.state = 0;
-            }
+            op_limit_ellipse(p, a[0], a[1], a[2], c.ignore_local());
             pc += 4;
             break;
         case OP_GLOBAL_APER:
-            if (!ignore_global) {
-                const double x = xval(p.x), y = xval(p.y);
-                const bool within = (x >= -GLOBAL_XY_LIMIT) && (x <= GLOBAL_XY_LIMIT) &&
-                                    (y >= -GLOBAL_XY_LIMIT) && (y <= GLOBAL_XY_LIMIT);
-                if (p.state > 0 && !within) p.state = -1;
-            }
+            op_global_aper(p, c.ignore_global());
             pc += 1;
             break;
         default:
@@ -631,33 +855,33 @@ XT_HD void run_element(PState<R> &p, const double *__restrict__ prog, int64_t pc
 
 // ---- monitor (monitors/particles_monitor.h:13-77) ---------------------------
 XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i,
-                          const PState<double> &p, int64_t store_at) {
+                          const PState<double> &p, int64_t at_element, int64_t store_at) {
     const xtm_particles &D = M.data;
-    D.p0c[store_at] = p.p0c;
+    D.p0c[store_at] = P.p0c[i];
     D.gamma0[store_at] = P.gamma0[i];
-    D.beta0[store_at] = p.beta0;
+    D.beta0[store_at] = P.beta0[i];
     D.s[store_at] = p.s;
     D.zeta[store_at] = p.zeta;
     D.x[store_at] = p.x;
     D.y[store_at] = p.y;
     D.px[store_at] = p.px;
     D.py[store_at] = p.py;
-    D.ptau[store_at] = p.ptau;
+    D.ptau[store_at] = P.ptau[i];
     D.delta[store_at] = p.delta;
     D.rpp[store_at] = p.rpp;
     D.rvv[store_at] = p.rvv;
     D.chi[store_at] = p.chi;
-    D.charge_ratio[store_at] = p.charge_ratio;
+    D.charge_ratio[store_at] = P.charge_ratio[i];
     D.weight[store_at] = P.weight[i];
-    D.ax[store_at] = p.ax;
-    D.ay[store_at] = p.ay;
+    D.ax[store_at] = P.ax[i];
+    D.ay[store_at] = P.ay[i];
     D.spin_x[store_at] = P.spin_x[i];
     D.spin_y[store_at] = P.spin_y[i];
     D.spin_z[store_at] = P.spin_z[i];
     D.anomalous_magnetic_moment[store_at] = P.anomalous_magnetic_moment[i];
     D.pdg_id[store_at] = P.pdg_id[i];
     D.particle_id[store_at] = P.particle_id[i];
-    D.at_element[store_at] = p.at_element;
+    D.at_element[store_at] = at_element;
     D.at_turn[store_at] = p.at_turn;
     D.state[store_at] = p.state;
     D.parent_particle_id[store_at] = P.parent_particle_id[i];
@@ -667,77 +891,113 @@ XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i
     D.rng_s4[store_at] = P.rng_s4[i];
 }
 
+/* Out of line and taking the state fields it stores as scalars, so that a
+   launch with monitoring off carries no cost for it. */
 XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int64_t i,
-                               const PState<double> p, int64_t pid) {
+                               double x, double px, double y, double py, double zeta,
+                               double delta, double rpp, double rvv, double s, double chi,
+                               int64_t at_turn, int64_t state, int64_t at_element) {
+    PState<double> p;
+    p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.delta = delta;
+    p.rpp = rpp; p.rvv = rvv; p.s = s; p.chi = chi; p.at_turn = at_turn; p.state = state;
+    const int64_t pid = P.particle_id[i];
     const int64_t n_turns_record = M.stop_at_turn - M.start_at_turn;
-    const int64_t at_turn = M.ebe_mode ? p.at_element : p.at_turn;
+    const int64_t at_turn_m = M.ebe_mode ? at_element : at_turn;
     if (M.n_repetitions == 1) {
-        if (at_turn >= M.start_at_turn && at_turn < M.stop_at_turn &&
+        if (at_turn_m >= M.start_at_turn && at_turn_m < M.stop_at_turn &&
             pid < M.part_id_end && pid >= M.part_id_start) {
-            monitor_store(M, P, i, p,
-                          n_turns_record * (pid - M.part_id_start) + at_turn - M.start_at_turn);
+            monitor_store(M, P, i, p, at_element,
+                          n_turns_record * (pid - M.part_id_start) + at_turn_m - M.start_at_turn);
         }
-    } else if (M.n_repetitions > 1 && at_turn >= M.start_at_turn) {
-        const int64_t i_frame = (at_turn - M.start_at_turn) / M.repetition_period;
+    } else if (M.n_repetitions > 1 && at_turn_m >= M.start_at_turn) {
+        const int64_t i_frame = (at_turn_m - M.start_at_turn) / M.repetition_period;
         if (i_frame < M.n_repetitions &&
-            at_turn >= M.start_at_turn + i_frame * M.repetition_period &&
-            at_turn < M.stop_at_turn + i_frame * M.repetition_period &&
+            at_turn_m >= M.start_at_turn + i_frame * M.repetition_period &&
+            at_turn_m < M.stop_at_turn + i_frame * M.repetition_period &&
             pid < M.part_id_end && pid >= M.part_id_start) {
-            monitor_store(M, P, i, p,
+            monitor_store(M, P, i, p, at_element,
                           n_turns_record * (M.part_id_end - M.part_id_start) * i_frame +
                               n_turns_record * (pid - M.part_id_start) +
-                              (at_turn - i_frame * M.repetition_period) - M.start_at_turn);
+                              (at_turn_m - i_frame * M.repetition_period) - M.start_at_turn);
         }
     }
 }
 
+#define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
+    monitor_record(M, P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta, p.rpp, p.rvv, p.s,      \
+                   p.chi, p.at_turn, p.state, (AT_ELEMENT))
 
 /* ---- one particle through [ele_start, ele_start+num_ele_track) for
    num_turns turns: the per-thread body of track_line (tracker.py:596-734)
    with the particle held in registers; monitor records only for active
-   particles as in the reference CPU context (track.h:33-50). */
+   particles as in the reference CPU context (track.h:33-50).
+   at_element: the reference adds 1 after every element the particle
+   survives and sets 0 at the end of a turn; here at_element = base + e at
+   element e, with base updated at the end of each turn.
+   run(p, m, e, c) tracks element e: the program interpreter, or the
+   line's generated code. */
+template <class Runner>
+XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner &run,
+                               const TrackCtx &c, int32_t num_turns, int32_t ele_start,
+                               int32_t num_ele_track, int32_t flag_end_turn_actions,
+                               int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor &M) {
+    if (!(P.state[i] > 0)) return;
+    PState<double> p;
+    p.x = P.x[i]; p.px = P.px[i]; p.y = P.y[i]; p.py = P.py[i];
+    p.zeta = P.zeta[i]; p.delta = P.delta[i];
+    p.rpp = P.rpp[i]; p.rvv = P.rvv[i]; p.s = P.s[i];
+    p.chi = P.chi[i];
+    p.at_turn = P.at_turn[i]; p.state = P.state[i];
+    p.inv_rvv = 1. / p.rvv;
+    refresh_flags(p);
+    const PRef m{&P, i};
+    const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
+    int64_t base = P.at_element[i] - ele_start;
+    int64_t e_lost = -1;
+    for (int32_t iturn = 0; iturn < num_turns; iturn++) {
+        if (flag_monitor == 1) XT_MONITOR_RECORD(base + ele_start);
+        for (int64_t e = ele_start; e < ele_stop; e++) {
+            if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
+            run(p, m, e, c);
+            if (p.state <= 0) { e_lost = e; break; }
+        }
+        if (e_lost >= 0) break;
+        if (flag_monitor == 2) XT_MONITOR_RECORD(base + ele_stop);
+        if (flag_end_turn_actions > 0) {
+            p.at_turn += 1;
+            base = -(int64_t)ele_start;   /* at_element = 0 at the next turn's start */
+            if (flag_reset_s > 0) p.s = 0.;
+        } else {
+            base += num_ele_track;
+        }
+    }
+    P.x[i] = p.x; P.px[i] = p.px; P.y[i] = p.y; P.py[i] = p.py;
+    P.zeta[i] = p.zeta; P.delta[i] = p.delta;
+    P.rpp[i] = p.rpp; P.rvv[i] = p.rvv; P.s[i] = p.s;
+    P.at_element[i] = base + (e_lost >= 0 ? e_lost : (int64_t)ele_start);
+    P.at_turn[i] = p.at_turn; P.state[i] = p.state;
+}
+
+/* the program interpreter as element runner */
+struct ProgramRunner {
+    const int64_t *__restrict__ entry;
+    const double *__restrict__ prog;
+    XT_HD void operator()(PState<double> &p, const PRef &m, int64_t e, const TrackCtx &c) const {
+        run_element(p, m, prog, entry[e], c);
+    }
+};
+
 XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__restrict__ entry,
                           const double *__restrict__ prog, double line_length,
                           int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
                           int32_t flag_end_turn_actions, int32_t flag_reset_s,
                           int32_t flag_monitor, const xtm_monitor &M, uint64_t flags) {
-    if (!(P.state[i] > 0)) return;
-    PState<double> p;
-    p.x = P.x[i]; p.px = P.px[i]; p.y = P.y[i]; p.py = P.py[i];
-    p.zeta = P.zeta[i]; p.delta = P.delta[i]; p.ptau = P.ptau[i];
-    p.rpp = P.rpp[i]; p.rvv = P.rvv[i]; p.s = P.s[i];
-    p.ax = P.ax[i]; p.ay = P.ay[i];
-    p.chi = P.chi[i]; p.charge_ratio = P.charge_ratio[i];
-    p.beta0 = P.beta0[i]; p.p0c = P.p0c[i];
-    p.at_element = P.at_element[i]; p.at_turn = P.at_turn[i]; p.state = P.state[i];
-    p.inv_rvv = 1. / p.rvv;
-    refresh_flags(p);
-    const int64_t pid = flag_monitor ? P.particle_id[i] : 0;
-    const double q0 = P.q0, t_sim = P.t_sim;
-    const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
-    for (int32_t iturn = 0; iturn < num_turns; iturn++) {
-        if (flag_monitor == 1) monitor_record(M, P, i, p, pid);
-        bool lost = false;
-        for (int64_t e = ele_start; e < ele_stop; e++) {
-            if (flag_monitor == 2) monitor_record(M, P, i, p, pid);
-            run_element(p, prog, entry[e], flags, line_length, q0, t_sim);
-            if (p.state <= 0) { lost = true; break; }
-            p.at_element += 1;
-        }
-        if (lost) break;
-        if (flag_monitor == 2) monitor_record(M, P, i, p, pid);
-        if (flag_end_turn_actions > 0) {
-            p.at_turn += 1;
-            p.at_element = 0;
-            if (flag_reset_s > 0) p.s = 0.;
-        }
-    }
-    P.x[i] = p.x; P.px[i] = p.px; P.y[i] = p.y; P.py[i] = p.py;
-    P.zeta[i] = p.zeta; P.delta[i] = p.delta; P.ptau[i] = p.ptau;
-    P.rpp[i] = p.rpp; P.rvv[i] = p.rvv; P.s[i] = p.s;
-    P.ax[i] = p.ax; P.ay[i] = p.ay;
-    P.at_element[i] = p.at_element; P.at_turn[i] = p.at_turn; P.state[i] = p.state;
+    const TrackCtx c{flags, line_length, P.q0, P.t_sim};
+    track_particle_with(P, i, ProgramRunner{entry, prog}, c, num_turns, ele_start, num_ele_track,
+                        flag_end_turn_actions, flag_reset_s, flag_monitor, M);
 }
+
+#undef XT_MONITOR_RECORD
 
 }  // namespace xt
 

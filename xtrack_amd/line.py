@@ -41,12 +41,22 @@ class Line:
                 tot += getattr(ee, 'length', 0.0)
         return tot
 
-    def build_tracker(self, _context=None, backend=None, device=None):
+    def build_tracker(self, _context=None, backend=None, device=None, compile='auto',
+                      waves_per_eu=0):
         """Build the device element table.  ``_context``/``device`` pick the
         HIP device (default ``cuda:0``); ``backend`` substitutes another
-        implementation of the two-method backend protocol."""
+        implementation of the two-method backend protocol.
+
+        ``compile``: generate and compile line code for this line (the
+        counterpart of the reference compiling its tracking kernel at this
+        point): 'auto' compiles when the line has few enough distinct
+        element records and otherwise uses the element-program interpreter,
+        True requires compiled code, False always interprets.
+        ``waves_per_eu``: occupancy target of the compiled kernel (0: the
+        library default)."""
         dev = device or (_context if isinstance(_context, str) else None) or 'cuda:0'
-        self.tracker = Tracker(self, backend=backend, device=dev)
+        self.tracker = Tracker(self, backend=backend, device=dev, compile=compile,
+                               waves_per_eu=waves_per_eu)
         return self.tracker
 
     def discard_tracker(self):

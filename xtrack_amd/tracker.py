@@ -57,7 +57,7 @@ class HIPBackend:
     """Device element table + kernel launches through libxtrack_amd.so."""
     space = 'device'
 
-    def __init__(self, device='cuda:0'):
+    def __init__(self, device='cuda:0', compile='auto', waves_per_eu=0):
         import torch
         self.lib = _native.load()
         if self.lib.xtm_device_count() < 1:
@@ -65,6 +65,10 @@ class HIPBackend:
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device('cuda:0')
+        if compile not in ('auto', True, False):
+            raise ValueError("compile must be 'auto', True or False")
+        self.compile = compile
+        self.waves_per_eu = int(waves_per_eu)
 
     def prepare_line(self, types, offsets, params, line_length):
         handle = ctypes.c_void_p()
@@ -72,7 +76,16 @@ class HIPBackend:
             types.ctypes.data, offsets.ctypes.data, params.ctypes.data,
             ctypes.c_int64(len(types)), ctypes.c_double(line_length),
             ctypes.c_int32(self.device.index), ctypes.byref(handle)))
-        return _LineHandle(self.lib, handle)
+        lh = _LineHandle(self.lib, handle)
+        if self.compile is not False:
+            st = self.lib.xtm_line_specialize(handle, self.waves_per_eu)
+            # too many distinct records: 'auto' keeps the interpreter
+            if st != 0 and not (st == _native.XTM_ERR_UNSUPPORTED and self.compile == 'auto'):
+                _native.check(st)
+        return lh
+
+    def is_compiled(self, handle):
+        return bool(self.lib.xtm_line_is_specialized(handle.ptr))
 
     def stream(self):
         import torch
@@ -115,9 +128,10 @@ class _LineHandle:
 
 
 class Tracker:
-    def __init__(self, line, backend=None, device='cuda:0'):
+    def __init__(self, line, backend=None, device='cuda:0', compile='auto', waves_per_eu=0):
         self.line = line
-        self.backend = backend if backend is not None else HIPBackend(device)
+        self.backend = (backend if backend is not None
+                        else HIPBackend(device, compile=compile, waves_per_eu=waves_per_eu))
         self.track_flags = TrackFlags()
         self.num_elements = len(line.elements)
         self.line_length = line.get_length()
