@@ -111,8 +111,13 @@ void polar_consts(double h, double s, double *ca, double *sa, double *t2, double
     *ca = c;
     *sa = sn;
     *t2 = 2 * s2 * s2;
-    const double ac = std::fabs(c);
-    *ica = (ac >= 0x1p-16 && ac <= 0x1p16) ? 1.0 / c : 0.0;
+    // ica = RN(1/ca) enables the kernel's reciprocal-division form of the
+    // polar drift; its range argument needs |h s| <= 0.1 (so ca >= 0.995,
+    // |sa| <= 0.1), |1/h| in [2, 2^400] and |sa| >= 2^-400.  Otherwise 0:
+    // the drift then runs with IEEE divisions only.
+    const double hs = std::fabs(h * s), rho = std::fabs(1 / h);
+    const bool in_range = hs <= 0.1 && rho >= 2 && rho <= 0x1p400 && std::fabs(sn) >= 0x1p-400;
+    *ica = in_range ? 1.0 / c : 0.0;
 }
 
 struct Builder {

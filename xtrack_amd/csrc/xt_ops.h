@@ -75,9 +75,10 @@ struct PState {
     int64_t at_turn, state;
     /* per-particle predicates of quantities that change only at load and in
        update_ptau (refresh_flags): the folded multipole kick may be used
-       (chi == 1, finite delta, rvv and 1/rvv, rvv != 0); rvv is in the
-       range of the reciprocal division (den_ok) */
-    bool fast_ok, rvv_ok;
+       (chi == 1, finite delta, rvv and 1/rvv, rvv != 0); delta and rvv
+       are in the ranges of the polar drift's reciprocal-division form
+       (delta in [-0.5, 1], rvv in [0.5, 2]) */
+    bool fast_ok, polar_ok;
 };
 
 /* the particle's slot in the arrays, for the fields not held in PState */
@@ -103,7 +104,7 @@ XT_HD double xval(double v) { return v; }
    correctly rounded quotient (P. Markstein, IBM J. Res. Dev. 34 (1990);
    J.-M. Muller et al., Handbook of Floating-Point Arithmetic, 2nd ed.,
    section 4.9).  Valid when a, b, y are finite, b normal and no
-   intermediate over/underflows; callers check ranges (div_ok). */
+   intermediate over/underflows; callers establish the ranges. */
 XT_HD double div_rcp(double a, double b, double y) {
     double q = a * y;
     double r = XT_FMA(-q, b, a);
@@ -112,17 +113,6 @@ XT_HD double div_rcp(double a, double b, double y) {
     return XT_FMA(r, y, q);
 }
 
-
-/* |v| in [2^-900, 2^900] (zero and non-finite values excluded) */
-XT_HD bool num_ok(double v) {
-    const double a = XT_FABS(v);
-    return a >= 0x1p-900 && a <= 0x1p900;
-}
-/* |v| in [2^-16, 2^16] */
-XT_HD bool den_ok(double v) {
-    const double a = XT_FABS(v);
-    return a >= 0x1p-16 && a <= 0x1p16;
-}
 
 /* ---- in-range IEEE reciprocal, quotient and square root (device) --------
    The general IEEE sequences for 1/b, a/b and sqrt(x) on CDNA are: scale
@@ -173,7 +163,25 @@ template <class R>
 XT_HD void refresh_flags(PState<R> &p) {
     p.fast_ok = (xval(p.chi) == 1.0) && xfinite(xval(p.delta)) && xfinite(xval(p.rvv)) &&
                 (xval(p.rvv) != 0.0) && xfinite(xval(p.inv_rvv));
-    p.rvv_ok = den_ok(xval(p.rvv));
+    p.polar_ok = xval(p.delta) >= -0.5 && xval(p.delta) <= 1.0 && xval(p.rvv) >= 0.5 &&
+                 xval(p.rvv) <= 2.0;
+}
+
+/* Lane condition of the polar drift's reciprocal-division form: with the
+   particle in polar_ok, the element's ica != 0 (|h s| <= 0.1, |1/h| in
+   [2, 2^400], |sa| >= 2^-400, checked by the builder) and |x| <= 1,
+   |px|, |py| <= 0.25, every operand stays in a safe range:
+   1 + delta in [0.5, 2], the radicand in [0.125, 4], pz in [0.35, 2],
+   ca - sa px/pz and 1 - n3/ca/pz in [0.92, 1.08], |n1| in
+   [2^-402, 2^400], |n3/ca/pz| <= 0.08, the dividend of the last two
+   divisions within [2^-404, 2^403].  The quotients by ca and pz of a
+   zero or tiny n3 (|n3| < 2^-900) may differ from the IEEE ones by the
+   sign of zero or one ulp of a number below 2^-890, which
+   1 - n3/ca/pz = 1 absorbs exactly. */
+template <class R>
+XT_HD bool polar_fast_ok(const PState<R> &p, const double ica) {
+    return p.polar_ok && (ica != 0.0) && xfabs(xval(p.x)) <= 1.0 &&
+           xfabs(xval(p.px)) <= 0.25 && xfabs(xval(p.py)) <= 0.25;
 }
 
 /* ---- drift maps --------------------------------------------------------- */
@@ -217,15 +225,12 @@ XT_HD_COLD double polar_dz_plain(double n1, double n3, double ca, double pz, dou
 
 /* Same value through div_rcp with the reciprocals ica = RN(1/ca) (element
    program), _pz = RN(1/pz) and inv_rvv = RN(1/rvv), which the drift has at
-   hand.  Range argument for exactness: with |n1|,|n3| in [2^-900, 2^900]
-   and ca, pz, den, rvv in [2^-16, 2^16] every dividend is within
-   [2^-948, 2^948], far from over- and underflow of the remainder.  Lanes
-   outside those ranges (including n3 == 0 for px == 0) take the literal
+   hand, on lanes where polar_fast_ok holds; other lanes take the literal
    form. */
 XT_HD double polar_dz(double n1, double n3, double ca, double ica, double pz, double _pz,
-                      double rvv, double inv_rvv, bool rvv_ok, double length) {
+                      double rvv, double inv_rvv, bool fast, double length) {
 #ifdef XTM_PLAIN_DIVISION
-    (void)ica; (void)_pz; (void)inv_rvv; (void)rvv_ok;
+    (void)ica; (void)_pz; (void)inv_rvv; (void)fast;
     const double delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
     return length - delta_ell / rvv;
 #else
@@ -234,9 +239,7 @@ XT_HD double polar_dz(double n1, double n3, double ca, double ica, double pz, do
     const double den = 1 - q4;
     const double delta_ell = q2 / den;
     double dz = length - div_rcp(delta_ell, rvv, inv_rvv);
-    const bool ok = rvv_ok && (ica != 0.0) && num_ok(n1) && num_ok(n3) && den_ok(pz) &&
-                    den_ok(den);
-    if (!ok) dz = polar_dz_plain(n1, n3, ca, pz, rvv, length);
+    if (!fast) dz = polar_dz_plain(n1, n3, ca, pz, rvv, length);
     return dz;
 #endif
 }
@@ -244,8 +247,8 @@ XT_HD double polar_dz(double n1, double n3, double ca, double ica, double pz, do
 /* operation-counting and other scalar types: the literal form */
 template <class R>
 XT_HD R polar_dz(R n1, R n3, double ca, double ica, R pz, R _pz, R rvv, R inv_rvv,
-                 bool rvv_ok, double length) {
-    (void)ica; (void)_pz; (void)inv_rvv; (void)rvv_ok;
+                 bool fast, double length) {
+    (void)ica; (void)_pz; (void)inv_rvv; (void)fast;
     const R delta_ell = n1 / ca / pz / (1 - n3 / ca / pz);
     return length - delta_ell / rvv;
 }
@@ -253,8 +256,14 @@ XT_HD R polar_dz(R n1, R n3, double ca, double ica, R pz, R _pz, R rvv, R inv_rv
 struct PolarOut { double x, px, y, dz; };
 
 /* track_magnet_drift.h:45-87 with IEEE operations throughout: the fallback
-   of polar_drift_ir, out of line (scalar arguments, four results in VGPRs). */
-XT_HD_COLD PolarOut polar_plain(const double x, const double px, const double y,
+   of polar_drift_ir, out of line by default (scalar arguments, four results
+   in VGPRs). */
+#ifdef XTM_POLAR_FALLBACK_INLINE
+XT_HD PolarOut polar_plain(
+#else
+XT_HD_COLD PolarOut polar_plain(
+#endif
+                                const double x, const double px, const double y,
                                 const double py, const double one_plus_delta,
                                 const double rvv, const double length, const double rho,
                                 const double ca, const double sa, const double t2) {
@@ -278,11 +287,9 @@ XT_HD_COLD PolarOut polar_plain(const double x, const double px, const double y,
 /* The polar drift with in-range arithmetic: sqrt_ir for pz, rcp_ir for 1/pz
    and 1/(ca - sa px/pz), div_rcp with the reciprocals ica = RN(1/ca),
    1/pz and inv_rvv for the five quotients by ca, pz and rvv, div_ir for the
-   division by 1 - n3/ca/pz.  Range argument: with the checked ranges
-   (radicand in [2^-32, 2^32], so pz in [2^-16, 2^16]; ca, ca - sa px/pz,
-   1 - n3/ca/pz and rvv in [2^-16, 2^16] in magnitude; |n1|, |n3| in
-   [2^-900, 2^900]) every dividend lies in [2^-948, 2^948].  Lanes outside
-   (including px == 0, which makes n3 zero) take polar_plain. */
+   division by 1 - n3/ca/pz, on lanes where polar_fast_ok holds (its
+   ranges are inside those of sqrt_ir, rcp_ir, div_ir and div_rcp); other
+   lanes take polar_plain. */
 __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double length,
                                                const double rho, const double ca,
                                                const double sa, const double t2,
@@ -309,9 +316,7 @@ __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double l
     const double den = 1 - q4;
     const double delta_ell = div_ir(q2, den);
     double dz = length - div_rcp(delta_ell, rvv, p.inv_rvv);
-    const bool ok = p.rvv_ok && (ica != 0.0) && rad >= 0x1p-32 && rad <= 0x1p32 &&
-                    den_ok(tden) && num_ok(n1) && num_ok(n3) && den_ok(den);
-    if (!ok) {
+    if (!polar_fast_ok(p, ica)) {
         const PolarOut o = polar_plain(x, px, y, py, one_plus_delta, rvv, length, rho, ca, sa, t2);
         new_x = o.x;
         new_px = o.px;
@@ -354,7 +359,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
     const R n1 = one_plus_delta * xr * sa;
     const R n3 = px * sa;
-    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv, p.rvv_ok, length);
+    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv, polar_fast_ok(p, ica), length);
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;
