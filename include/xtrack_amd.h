@@ -215,13 +215,14 @@ int32_t xtm_line_is_specialized(const xtm_line *line);
  * xtm_line_specialize compiles, written to buf (capacity cap, NUL
  * terminated, *len = full length), the record index of each element
  * (rec_of_elem, num_elements entries, may be NULL), and the element
- * program the generated code refers to (xtm_line_program_size doubles).
- * Works on host-only handles; the CPU tests compile the source for the
- * host. */
+ * program the generated code refers to (xtm_line_program_size doubles)
+ * with its per-element entry offsets (num_elements int64).  Works on
+ * host-only handles; the CPU tests compile the source for the host. */
 int32_t xtm_line_generate_source(const xtm_line *line, int32_t waves_per_eu, char *buf,
                                  int64_t cap, int64_t *len, int32_t *rec_of_elem);
 int64_t xtm_line_program_size(const xtm_line *line);
 int32_t xtm_line_copy_program(const xtm_line *line, double *out);
+int32_t xtm_line_copy_entry(const xtm_line *line, int64_t *out);
 
 /* Track particles in place, asynchronously on `stream`.  Same arguments and
  * semantics as the reference kernel `track_line` (xtrack/tracker.py:546-564):

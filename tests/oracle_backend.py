@@ -148,6 +148,8 @@ def generated_host_line(types, offsets, params, line_length):
                                                    None))
         prog = np.zeros(lib.xtm_line_program_size(h), dtype=np.float64)
         _native.check(lib.xtm_line_copy_program(h, prog.ctypes.data))
+        entry = np.zeros(max(len(types), 1), dtype=np.int64)
+        _native.check(lib.xtm_line_copy_entry(h, entry.ctypes.data))
     finally:
         lib.xtm_line_destroy(h)
     src = buf.value.decode()
@@ -165,28 +167,29 @@ def generated_host_line(types, offsets, params, line_length):
         glib = ctypes.CDLL(so)
         c = ctypes
         glib.xtm_line_host.restype = None
-        glib.xtm_line_host.argtypes = [c.c_void_p, c.c_void_p, c.c_double,
+        glib.xtm_line_host.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_double,
                                        c.POINTER(_native.XtmParticles), c.c_int32, c.c_int32,
                                        c.c_int32, c.c_int32, c.c_int32, c.c_int32,
                                        c.POINTER(_native.XtmMonitor), c.c_uint64]
         _gen_cache[key] = glib
-    return _gen_cache[key], rec, prog, src
+    return _gen_cache[key], rec, prog, src, entry
 
 
 class GeneratedHostBackend(OracleBackend):
     """Same protocol, running the library's generated line code on the host."""
 
     def prepare_line(self, types, offsets, params, line_length):
-        glib, rec, prog, _ = generated_host_line(types, offsets, params, line_length)
-        return glib, rec, prog, line_length
+        glib, rec, prog, _, entry = generated_host_line(types, offsets, params, line_length)
+        return glib, rec, entry, prog, line_length
 
     def track_line(self, handle, particles, num_turns, ele_start, num_ele_track,
                    flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor,
                    monitor, track_flags):
-        glib, rec, prog, line_length = handle
+        glib, rec, entry, prog, line_length = handle
         pst = _native.make_particles_struct(particles)
         mst = _native.make_monitor_struct(monitor) if monitor is not None else None
-        glib.xtm_line_host(rec.ctypes.data, prog.ctypes.data, line_length, ctypes.byref(pst),
+        glib.xtm_line_host(rec.ctypes.data, entry.ctypes.data, prog.ctypes.data, line_length,
+                           ctypes.byref(pst),
                            int(num_turns), int(ele_start), int(num_ele_track),
                            int(flag_end_turn_actions), int(flag_reset_s_at_end_turn),
                            int(flag_monitor), ctypes.byref(mst) if mst is not None else None,

@@ -69,7 +69,7 @@ def test_source_shape():
     from xtrack_amd import lattices
     line = lattices.collider_ring(k2f=0.03, k2d=-0.06)
     t, o, pp = build_raw_table(line.elements)
-    _, rec, prog, src = generated_host_line(t, o, pp, line.get_length())
+    _, rec, prog, src, _ = generated_host_line(t, o, pp, line.get_length())
     assert len(np.unique(rec)) == 10
     assert src.count('XT_GEN_FN void rec_') == 10
     assert 'xt::op_yoshida4(p' in src and 'xt::op_limit_rect(p' in src

@@ -134,6 +134,8 @@ def load():
     lib.xtm_line_program_size.argtypes = [c.c_void_p]
     lib.xtm_line_copy_program.restype = c.c_int32
     lib.xtm_line_copy_program.argtypes = [c.c_void_p, c.c_void_p]
+    lib.xtm_line_copy_entry.restype = c.c_int32
+    lib.xtm_line_copy_entry.argtypes = [c.c_void_p, c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

@@ -204,27 +204,30 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
                   waves_per_eu > 0 ? waves_per_eu : 1);
     s += std::string("extern \"C\" __global__ __launch_bounds__(256) ") +
          (waves_per_eu > 0 ? occ : "") +
-         "\nvoid xtm_line_kernel(const int32_t *__restrict__ rec, const double *__restrict__ prog,\n"
-         "    double line_length, xtm_particles P, int32_t num_turns, int32_t ele_start,\n"
-         "    int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,\n"
-         "    int32_t flag_monitor, xtm_monitor M, uint64_t flags) {\n"
+         "\nvoid xtm_line_kernel(const int32_t *__restrict__ rec, const int64_t *__restrict__ entry,\n"
+         "    const double *__restrict__ prog, double line_length, xtm_particles P,\n"
+         "    int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
+         "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
+         "    xtm_monitor M, uint64_t flags) {\n"
          "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
-         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, prog}, c, num_turns, ele_start,\n"
-         "                            num_ele_track, flag_end_turn_actions, flag_reset_s,\n"
-         "                            flag_monitor, M);\n"
+         "    // generated code for chi == 1 particles, the interpreter for the others\n"
+         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, prog}, xt::ProgramRunner{entry, prog},\n"
+         "                            c, num_turns, ele_start, num_ele_track,\n"
+         "                            flag_end_turn_actions, flag_reset_s, flag_monitor, M);\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over all particles\n"
-         "extern \"C\" void xtm_line_host(const int32_t *rec, const double *prog, double line_length,\n"
-         "    const xtm_particles *P, int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
-         "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
-         "    const xtm_monitor *M, uint64_t flags) {\n"
+         "extern \"C\" void xtm_line_host(const int32_t *rec, const int64_t *entry, const double *prog,\n"
+         "    double line_length, const xtm_particles *P, int32_t num_turns, int32_t ele_start,\n"
+         "    int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,\n"
+         "    int32_t flag_monitor, const xtm_monitor *M, uint64_t flags) {\n"
          "    xtm_monitor m0 = {};\n"
          "    const xt::TrackCtx c{flags, line_length, P->q0, P->t_sim};\n"
          "    for (int64_t i = 0; i < P->capacity; i++)\n"
-         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, prog}, c, num_turns,\n"
+         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, prog},\n"
+         "                                xt::ProgramRunner{entry, prog}, c, num_turns,\n"
          "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
          "                                flag_reset_s, flag_monitor, M ? *M : m0);\n"
          "}\n"

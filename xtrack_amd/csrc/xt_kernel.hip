@@ -323,6 +323,12 @@ int32_t xtm_line_generate_source(const xtm_line *L, int32_t waves_per_eu, char *
 
 int64_t xtm_line_program_size(const xtm_line *L) { return L ? (int64_t)L->prog.size() : -1; }
 
+int32_t xtm_line_copy_entry(const xtm_line *L, int64_t *out) {
+    if (!L || !out) return fail(XTM_ERR_ARG, "xtm_line_copy_entry: null argument");
+    std::memcpy(out, L->entry.data(), L->entry.size() * sizeof(int64_t));
+    return XTM_OK;
+}
+
 int32_t xtm_line_copy_program(const xtm_line *L, double *out) {
     if (!L || !out) return fail(XTM_ERR_ARG, "xtm_line_copy_program: null argument");
     std::memcpy(out, L->prog.data(), L->prog.size() * sizeof(double));
@@ -362,13 +368,14 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
     if (L->kernel) {
         xtm_particles Pv = *P;
         const int32_t *rec = L->d_rec;
+        const int64_t *entry = L->d_entry;
         const double *prog = L->d_prog;
         double line_length = L->line_length;
         int32_t a_turns = num_turns, a_start = ele_start, a_n = num_ele_track;
         int32_t a_end = flag_end_turn_actions, a_reset = flag_reset_s_at_end_turn;
         int32_t a_mon = flag_monitor;
         uint64_t a_flags = track_flags;
-        void *args[] = {&rec, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
+        void *args[] = {&rec, &entry, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
                         &a_end, &a_reset, &a_mon, &M, &a_flags};
         e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
                                   (hipStream_t)stream, args, nullptr);

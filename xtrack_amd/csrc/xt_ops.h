@@ -940,18 +940,21 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
    survives and sets 0 at the end of a turn; here at_element = base + e at
    element e, with base updated at the end of each turn.
    run(p, m, e, c) tracks element e: the program interpreter, or the
-   line's generated code. */
-template <class Runner>
-XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner &run,
+   line's generated code.
+   CHI1: the particle's chi is exactly 1.0 and the element code sees the
+   constant, so products with chi fold away (chi * k == k exactly) instead
+   of being evaluated, or hoisted out of the loops as per-particle values
+   that occupy registers for the whole launch. */
+template <bool CHI1, class Runner>
+XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &run,
                                const TrackCtx &c, int32_t num_turns, int32_t ele_start,
                                int32_t num_ele_track, int32_t flag_end_turn_actions,
                                int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor &M) {
-    if (!(P.state[i] > 0)) return;
     PState<double> p;
     p.x = P.x[i]; p.px = P.px[i]; p.y = P.y[i]; p.py = P.py[i];
     p.zeta = P.zeta[i]; p.delta = P.delta[i];
     p.rpp = P.rpp[i]; p.rvv = P.rvv[i]; p.s = P.s[i];
-    p.chi = P.chi[i];
+    p.chi = CHI1 ? 1.0 : P.chi[i];
     p.at_turn = P.at_turn[i]; p.state = P.state[i];
     p.inv_rvv = 1. / p.rvv;
     refresh_flags(p);
@@ -992,13 +995,31 @@ struct ProgramRunner {
     }
 };
 
+/* run_chi1 serves particles with chi == 1.0 (track_particle_loop<true>),
+   run_other all others */
+template <class Runner1, class RunnerOther>
+XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner1 &run_chi1,
+                               const RunnerOther &run_other, const TrackCtx &c,
+                               int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
+                               int32_t flag_end_turn_actions, int32_t flag_reset_s,
+                               int32_t flag_monitor, const xtm_monitor &M) {
+    if (!(P.state[i] > 0)) return;
+    if (P.chi[i] == 1.0)
+        track_particle_loop<true>(P, i, run_chi1, c, num_turns, ele_start, num_ele_track,
+                                  flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+    else
+        track_particle_loop<false>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
+                                   flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+}
+
 XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__restrict__ entry,
                           const double *__restrict__ prog, double line_length,
                           int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
                           int32_t flag_end_turn_actions, int32_t flag_reset_s,
                           int32_t flag_monitor, const xtm_monitor &M, uint64_t flags) {
     const TrackCtx c{flags, line_length, P.q0, P.t_sim};
-    track_particle_with(P, i, ProgramRunner{entry, prog}, c, num_turns, ele_start, num_ele_track,
+    const ProgramRunner run{entry, prog};
+    track_particle_with(P, i, run, run, c, num_turns, ele_start, num_ele_track,
                         flag_end_turn_actions, flag_reset_s, flag_monitor, M);
 }
 
