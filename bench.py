@@ -44,10 +44,11 @@ NEMITT = 3.75e-6
 SIGMA_Z = 0.0755
 SIGMA_DELTA = 1.13e-4
 FLOP_PER_CALL = 20     # weight of one elementary-function call
-# particle state moved per particle and launch: 19 fields of 8 bytes read
-# (x px y py zeta delta ptau rpp rvv s ax ay chi charge_ratio beta0 p0c
-# at_element at_turn state), 15 written back (all but chi charge_ratio beta0 p0c)
-STATE_BYTES_PER_PARTICLE = (19 + 15) * 8
+# particle state moved per particle and launch: 13 fields of 8 bytes read
+# (x px y py zeta delta rpp rvv s chi at_element at_turn state), 12 written
+# back (all but chi); the cavity and bend edges read/write a few more fields
+# in place (ptau beta0 charge_ratio p0c ax ay), counted by the traffic runs
+STATE_BYTES_PER_PARTICLE = (13 + 12) * 8
 COPY_MIB = 256
 
 
@@ -59,7 +60,7 @@ def parse():
     ap.add_argument('--turns-per-step', type=int, default=1)
     ap.add_argument('--particles', type=int, default=1_000_000)
     ap.add_argument('--cpu-sample-particles', type=int, default=8192)
-    ap.add_argument('--cpu-sample-turns', type=int, default=20)
+    ap.add_argument('--cpu-sample-turns', type=int, default=24)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--traffic', choices=('measure', 'none'), default='measure',
                     help='measure HBM traffic with two rocprofv3 --pmc child runs')
@@ -78,7 +79,9 @@ def parse():
 def _read_counter(outdir, counter):
     """(copy_value, track_value) of `counter` from a rocprofv3 csv run: the
     copy is the calibration dispatch (kernel name holds 'neg'), the tracking
-    value is that of the last track_kernel dispatch (the timed launch)."""
+    value is that of the last tracking dispatch (the timed launch) of either
+    kernel: compiled line code (xtm_line_kernel) or the interpreter
+    (track_kernel)."""
     files = glob.glob(os.path.join(outdir, '**', '*counter_collection.csv'), recursive=True)
     if not files:
         raise RuntimeError(f'no counter_collection.csv under {outdir}')
@@ -95,7 +98,7 @@ def _read_counter(outdir, counter):
                 per_dispatch[key] = (name, old[1] + val)
     items = sorted(per_dispatch.items(), key=lambda kv: kv[0][1])
     copy = [v for _, (nm, v) in items if 'neg' in nm]
-    track = [v for _, (nm, v) in items if 'track_kernel' in nm]
+    track = [v for _, (nm, v) in items if 'xtm_line_kernel' in nm or 'track_kernel' in nm]
     if not copy or not track:
         raise RuntimeError(f'{counter}: calibration or tracking dispatch not found')
     return max(copy), track[-1]

@@ -39,23 +39,27 @@ std::string lit(double v) {
     return s;
 }
 
-std::string lits(const double *a, int n) {
-    std::string s;
-    for (int k = 0; k < n; k++) {
-        s += ", ";
-        s += lit(a[k]);
-    }
-    return s;
-}
-
 const int kNargs[OP_NUM] = XT_OP_NARGS_INIT;
 
 struct Gen {
     const std::vector<double> &prog;
+    const int64_t base;   // program offset of the record
+    const bool loads;     // constants as loads rp[k] instead of literals
     std::string out;
     std::string err;
 
-    explicit Gen(const std::vector<double> &p) : prog(p) {}
+    Gen(const std::vector<double> &p, int64_t b, bool l) : prog(p), base(b), loads(l) {}
+
+    // constant at absolute program index i
+    std::string c(int64_t i) const {
+        if (loads) return "rp[" + std::to_string(i - base) + "]";
+        return lit(prog[i]);
+    }
+    std::string cs(int64_t i, int n) const {
+        std::string s;
+        for (int k = 0; k < n; k++) s += ", " + c(i + k);
+        return s;
+    }
 
     void line(const std::string &s) { out += "    " + s + "\n"; }
 
@@ -64,32 +68,31 @@ struct Gen {
     }
 
     // one folded Horner pass from the coefficient block at `fp`
-    bool kick_pass(int64_t fp, double w) {
+    bool kick_pass(int64_t fp, const std::string &w) {
         if (!in_range(fp, 1)) return false;
         const int64_t top = wbits(prog[fp]);
         if (top < 0 || !in_range(fp, 2 * top + 3)) return false;
-        const double *a = &prog[fp + 1];
-        const double *b = &prog[fp + 2 + top];
+        const int64_t ia = fp + 1, ib = fp + 2 + top;
         line("    {");
-        line("        double dpx = " + lit(a[top]) + ", dpy = " + lit(b[top]) + ";");
+        line("        double dpx = " + c(ia + top) + ", dpy = " + c(ib + top) + ";");
         for (int64_t k = top - 1; k >= 0; k--) {
-            line("        xt::horner_step(dpx, dpy, x, y, " + lit(a[k]) + ", " + lit(b[k]) + ", " +
-                 (a[k] != 0.0 ? "true" : "false") + ", " + (b[k] != 0.0 ? "true" : "false") +
-                 ");");
+            line("        xt::horner_step(dpx, dpy, x, y, " + c(ia + k) + ", " + c(ib + k) + ", " +
+                 (prog[ia + k] != 0.0 ? "true" : "false") + ", " +
+                 (prog[ib + k] != 0.0 ? "true" : "false") + ");");
         }
-        line("        xt::kick_pass_apply(p, " + lit(w) + ", dpx, dpy);");
+        line("        xt::kick_pass_apply(p, " + w + ", dpx, dpy);");
         line("    }");
         return true;
     }
 
     bool kick(int64_t pc) {
         const double *a = &prog[pc + 1];
-        const double w = a[0];
+        const std::string w = c(pc + 1);
         const int64_t desc = wbits(a[1]);
         const int64_t mask = wbits(a[2]);
         if (!in_range(desc, KD_ARR)) return false;
         line("if (!xt::kick_sane(p)) {");
-        line("    xt::kick_literal_apply(p, " + lit(w) + ", prog + " + std::to_string(desc) + ");");
+        line("    xt::kick_literal_apply(p, " + w + ", prog + " + std::to_string(desc) + ");");
         line("} else {");
         line("    const double x = p.x, y = p.y;");
         for (int k = 0; k < 3; k++) {
@@ -99,7 +102,7 @@ struct Gen {
         if (mask & (8 | 16 | 32)) {
             line(std::string("    xt::kick_curvature(p, x, y, ") + ((mask & 8) ? "true" : "false") +
                  ", " + ((mask & 16) ? "true" : "false") + ", " + ((mask & 32) ? "true" : "false") +
-                 lits(a + 3, 3) + ");");
+                 cs(pc + 4, 3) + ");");
         }
         line("}");
         return true;
@@ -113,30 +116,30 @@ struct Gen {
             if (op == OP_END) return true;
             const int n = kNargs[op];
             if (!in_range(pc + 1, n)) return false;
-            const double *a = &prog[pc + 1];
+            const int64_t a = pc + 1;
             const std::string at = "prog + " + std::to_string(pc + 1);
             switch (op) {
-            case OP_DRIFT_EXP: line("xt::drift_expanded(p" + lits(a, 1) + ");"); break;
-            case OP_DRIFT_EXACT: line("xt::drift_exact(p" + lits(a, 1) + ");"); break;
-            case OP_POLAR: line("xt::polar_drift(p" + lits(a, 6) + ");"); break;
-            case OP_DIPQUAD: line("xt::dipole_quad(p" + lits(a, 4) + ");"); break;
-            case OP_YOSHIDA4_BEND: line("xt::op_yoshida4(p" + lits(a, 14) + ");"); break;
+            case OP_DRIFT_EXP: line("xt::drift_expanded(p" + cs(a, 1) + ");"); break;
+            case OP_DRIFT_EXACT: line("xt::drift_exact(p" + cs(a, 1) + ");"); break;
+            case OP_POLAR: line("xt::polar_drift(p" + cs(a, 6) + ");"); break;
+            case OP_DIPQUAD: line("xt::dipole_quad(p" + cs(a, 4) + ");"); break;
+            case OP_YOSHIDA4_BEND: line("xt::op_yoshida4(p" + cs(a, 14) + ");"); break;
             case OP_YOSHIDA6_BEND: line("xt::op_yoshida6(p, " + at + ");"); break;
             case OP_KICK:
                 if (!kick(pc)) return false;
                 break;
-            case OP_EDGE_AXAY: line("xt::op_edge_axay(p, m" + lits(a, 2) + ");"); break;
+            case OP_EDGE_AXAY: line("xt::op_edge_axay(p, m" + cs(a, 2) + ");"); break;
             case OP_EDGE_CLEAR_AXAY: line("xt::op_edge_clear_axay(m);"); break;
-            case OP_EDGE_LINEAR: line("xt::op_edge_linear(p" + lits(a, 2) + ");"); break;
+            case OP_EDGE_LINEAR: line("xt::op_edge_linear(p" + cs(a, 2) + ");"); break;
             case OP_RF_KICK:
                 line("xt::op_rf_kick(p, m, " + at +
                      ", c.line_length, c.q0, c.t_sim, c.kill_cavity());");
                 break;
             case OP_LIMIT_RECT:
-                line("xt::op_limit_rect(p" + lits(a, 4) + ", c.ignore_local());");
+                line("xt::op_limit_rect(p" + cs(a, 4) + ", c.ignore_local());");
                 break;
             case OP_LIMIT_ELLIPSE:
-                line("xt::op_limit_ellipse(p" + lits(a, 3) + ", c.ignore_local());");
+                line("xt::op_limit_ellipse(p" + cs(a, 3) + ", c.ignore_local());");
                 break;
             case OP_GLOBAL_APER: line("xt::op_global_aper(p, c.ignore_global());"); break;
             default: return false;
@@ -171,28 +174,38 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
     s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";
     s += "#else\n#define XT_GEN_FN static inline\n#endif\n\n";
     s += "namespace xt_gen {\n";
+#ifdef XTM_CODEGEN_CONST_LOADS
+    const bool loads = true;
+#else
+    const bool loads = false;
+#endif
     for (auto &kv : index) {
-        Gen g(prog);
+        Gen g(prog, kv.first, loads);
         if (!g.record(kv.first)) {
             err = "cannot generate code for the program at offset " + std::to_string(kv.first);
             return XTM_ERR_ARG;
         }
         s += "XT_GEN_FN void rec_" + std::to_string(kv.second) +
              "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
-             "    const xt::TrackCtx &c) {\n"
-             "    (void)m; (void)prog; (void)c;\n";
+             "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n"
+             "    (void)m; (void)prog; (void)rp; (void)c;\n";
         s += g.out;
         s += "}\n\n";
     }
     s += "struct Runner {\n"
          "    const int32_t *__restrict__ rec;\n"
+         "    const int64_t *__restrict__ entry;\n"
          "    const double *__restrict__ prog;\n"
          "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, int64_t e,\n"
-         "                          const xt::TrackCtx &c) const {\n"
-         "        switch (rec[e]) {\n";
+         "                          const xt::TrackCtx &c) const {\n";
+    if (loads)
+        s += "        const double *__restrict__ rp = prog + entry[e];\n";
+    else
+        s += "        const double *__restrict__ rp = prog;\n";
+    s += "        switch (rec[e]) {\n";
     for (int32_t r = 0; r < k; r++)
         s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
-             "(p, m, prog, c); break;\n";
+             "(p, m, prog, rp, c); break;\n";
     s += "        default: p.state = -999; break;\n"
          "        }\n"
          "    }\n"
@@ -213,7 +226,8 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
          "    // generated code for chi == 1 particles, the interpreter for the others\n"
-         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, prog}, xt::ProgramRunner{entry, prog},\n"
+         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, entry, prog},\n"
+         "                            xt::ProgramRunner{entry, prog},\n"
          "                            c, num_turns, ele_start, num_ele_track,\n"
          "                            flag_end_turn_actions, flag_reset_s, flag_monitor, M);\n"
          "}\n"
@@ -226,7 +240,7 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    xtm_monitor m0 = {};\n"
          "    const xt::TrackCtx c{flags, line_length, P->q0, P->t_sim};\n"
          "    for (int64_t i = 0; i < P->capacity; i++)\n"
-         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, prog},\n"
+         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, entry, prog},\n"
          "                                xt::ProgramRunner{entry, prog}, c, num_turns,\n"
          "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
          "                                flag_reset_s, flag_monitor, M ? *M : m0);\n"
