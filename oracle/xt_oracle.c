@@ -1036,7 +1036,6 @@ int32_t xto_track_line(const int32_t *types, const int64_t *offsets, const doubl
         return XTM_ERR_ARG;
     if (flag_monitor && !monitor) return XTM_ERR_ARG;
     if ((track_flags >> XTM_FLAG_SR_TAPER) & 1) return XTM_ERR_UNSUPPORTED;
-    if ((track_flags >> XTM_FLAG_BACKTRACK) & 1) return XTM_ERR_UNSUPPORTED;
     const int64_t capacity = P->capacity;
     if (num_threads < 1) num_threads = 1;
 #ifdef _OPENMP
@@ -1063,12 +1062,25 @@ int32_t xto_track_line(const int32_t *types, const int64_t *offsets, const doubl
         if (part_id >= capacity) continue;
         LP lpart = {P, part_id, end_id, track_flags, line_length};
         int64_t isactive = (part_id < end_id) ? check_is_active(&lpart) : 0;
+        const int backtrack = (int)((track_flags >> XTM_FLAG_BACKTRACK) & 1);
         for (int64_t iturn = 0; iturn < num_turns; iturn++) {
             if (!isactive) break;
             int64_t const ele_stop = ele_start + num_ele_track;
-            if (flag_monitor == 1) monitor_track(monitor, &lpart);
-            for (int64_t elem_idx = ele_start; elem_idx >= ele_start && elem_idx < ele_stop;
-                 elem_idx++) {
+            int64_t elem_idx, increm;
+            /* tracker.py:629-642: backtracking walks the elements in reverse
+               and moves to the previous turn before the element loop */
+            if (backtrack) {
+                elem_idx = ele_stop - 1;
+                increm = -1;
+                if (flag_end_turn_actions > 0)
+                    increment_at_turn_backtrack(&lpart, flag_reset_s_at_end_turn, line_length,
+                                                num_elements);
+            } else {
+                if (flag_monitor == 1) monitor_track(monitor, &lpart);
+                elem_idx = ele_start;
+                increm = 1;
+            }
+            for (; elem_idx >= ele_start && elem_idx < ele_stop; elem_idx += increm) {
                 if (flag_monitor == 2) monitor_track(monitor, &lpart);
                 int st = track_element(&lpart, types[elem_idx], params + offsets[elem_idx]);
                 if (st) {
@@ -1079,10 +1091,13 @@ int32_t xto_track_line(const int32_t *types, const int64_t *offsets, const doubl
                 }
                 isactive = check_is_active(&lpart);
                 if (!isactive) break;
-                increment_at_element(&lpart, 1);
+                increment_at_element(&lpart, backtrack ? -1 : 1);
             }
             if (flag_monitor == 2) monitor_track(monitor, &lpart);
-            if (flag_end_turn_actions > 0) {
+            /* tracker.py:719-731 */
+            if (backtrack) {
+                if (flag_monitor == 1) monitor_track(monitor, &lpart);
+            } else if (flag_end_turn_actions > 0) {
                 if (isactive) increment_at_turn(&lpart, flag_reset_s_at_end_turn);
             }
         }

@@ -16,15 +16,16 @@ extern "C" int32_t xth_track_line(const int32_t *types, const int64_t *offsets,
                                   int32_t num_ele_track, int32_t flag_end_turn_actions,
                                   int32_t flag_reset_s, int32_t flag_monitor,
                                   const xtm_monitor *M, uint64_t flags) {
-    std::vector<int64_t> entry;
+    std::vector<int64_t> entry, entry_bt;
     std::vector<double> prog;
     std::string err;
-    int st = xt_build_program(types, offsets, params, n, entry, prog, err);
+    int st = xt_build_program(types, offsets, params, n, entry, prog, err, &entry_bt);
     if (st) return st;
     xtm_monitor m0 = {};
     const xtm_monitor &MM = M ? *M : m0;
+    const int64_t *ent = (flags & 1) ? entry_bt.data() : entry.data();
     for (int64_t i = 0; i < P->capacity; i++)
-        xt::track_particle(*P, i, entry.data(), prog.data(), line_length, num_turns, ele_start,
+        xt::track_particle(*P, i, ent, prog.data(), line_length, n, num_turns, ele_start,
                            num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor,
                            MM, flags);
     return 0;

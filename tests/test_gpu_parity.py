@@ -193,3 +193,23 @@ def test_monitors_bit_exact():
             a, b = getattr(mg, ff), getattr(mc, ff)
             assert a.shape == b.shape
             assert np.array_equal(a, b, equal_nan=True), (mode, ff)
+
+
+@pytest.mark.parametrize('kw', [dict(num_turns=2), dict(ele_start=3, ele_stop=11),
+                                dict(num_turns=1, turn_by_turn_monitor='ONE_TURN_EBE')])
+def test_backtracking_bit_exact(kw):
+    """Forward two turns, then backtrack (the interpreter runs the
+    backtracking programs whichever kernel tracks forward)."""
+    print(f'[backtrack] {kw}', flush=True)
+    p = random_particles(1500, seed=8, scale=5e-3)
+    lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=2), p)
+    lg.track(pg, num_turns=2)
+    lc.track(pc, num_turns=2)
+    lg.track(pg, backtrack=True, **kw)
+    lc.track(pc, backtrack=True, **kw)
+    bad = compare_exact(pg, pc)
+    assert not bad, bad
+    if 'turn_by_turn_monitor' in kw:
+        for ff in ('x', 'px', 'zeta', 'at_element', 'at_turn', 'state'):
+            assert np.array_equal(np.asarray(getattr(lg.record_last_track, ff)),
+                                  getattr(lc.record_last_track, ff), equal_nan=True), ff

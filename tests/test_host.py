@@ -193,7 +193,29 @@ def test_track_argument_errors():
         line.track(p, num_elements=3, ele_stop=2)
     with pytest.raises(ValueError):
         line.track(p, num_elements=3, num_turns=2)
-    with pytest.raises(NotImplementedError):
-        line.track(p, backtrack=True)
+    with pytest.raises(ValueError):
+        line.track(p, backtrack='bogus')
     with pytest.raises(ValueError):
         line.track(p, turn_by_turn_monitor='bogus')
+
+
+def test_backtrack_sets_flag_for_the_call_only():
+    """tracker.py:1222-1236: XS_FLAG_BACKTRACK (bit 0) is set while the
+    request runs and restored afterwards; segmentation is unchanged."""
+    line = xt.Line(elements=[xt.Drift(length=1.0) for _ in range(10)])
+    rec = _Recorder()
+    seen = []
+    orig = rec.track_line
+
+    def spy(h, p, num_turns, ele_start, num_ele_track, end_turn, reset_s, flag_monitor,
+            monitor, flags):
+        seen.append(int(flags) & 1)
+        orig(h, p, num_turns, ele_start, num_ele_track, end_turn, reset_s, flag_monitor,
+             monitor, flags)
+    rec.track_line = spy
+    line.build_tracker(backend=rec)
+    line.track(xt.Particles(x=[0.0]), num_turns=2, backtrack=True)
+    assert seen == [1, 1]
+    assert rec.calls == [(1, 0, 10, True, 0), (1, 0, 10, True, 0)]
+    line.track(xt.Particles(x=[0.0]), num_turns=1)
+    assert seen[-1] == 0 and not line.track_flags.XS_FLAG_BACKTRACK

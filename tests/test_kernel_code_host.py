@@ -114,3 +114,19 @@ def test_reciprocal_division_is_correctly_rounded():
     b3 = np.nextafter(2.0 ** rng.integers(-10, 10, n).astype(float), 0)
     a3 = rng.uniform(-4, 4, n)
     assert lib.xth_check_div_rcp(a3.ctypes.data, b3.ctypes.data, n) == 0
+
+
+@pytest.mark.parametrize('kw', [dict(num_turns=2), dict(ele_start=3, ele_stop=11),
+                                dict(num_turns=1, turn_by_turn_monitor='ONE_TURN_EBE')])
+def test_backtracking(kw):
+    """Backtracking programs and the reversed element loop against the
+    oracle, after a forward pass; losses included."""
+    p = random_particles(400, seed=8, scale=5e-3)
+    lh, lc, ph, pc = _run(lambda: mixed_line(n_cells=2), p, num_turns=2)
+    lh.track(ph, backtrack=True, **kw)
+    lc.track(pc, backtrack=True, **kw)
+    assert not compare_exact(ph, pc)
+    if 'turn_by_turn_monitor' in kw:
+        for ff in ('x', 'px', 'zeta', 'at_element', 'at_turn', 'state'):
+            assert np.array_equal(getattr(lh.record_last_track, ff),
+                                  getattr(lc.record_last_track, ff), equal_nan=True), ff

@@ -399,3 +399,86 @@ def test_one_over_factorial_table_is_correctly_rounded():
     assert len(vals) == 20
     for n, v in enumerate(vals):
         assert float.fromhex(v) == float(Fraction(1, math.factorial(n)))
+
+
+# ---------------------------------------------------------------------------
+# backtracking (xtrack tests/test_elements.py:429-474, tests/test_magnet.py:96-108,
+# tests/test_tracker.py:528-553)
+_BT_ELEMENTS = [
+    ('drift', lambda: xt.Drift(length=1.0), 1e-14),
+    ('drift_exact', lambda: xt.Drift(length=1.0, model='exact'), 1e-14),
+    ('multipole', lambda: xt.Multipole(knl=[2, 3]), 1e-14),
+    ('cavity', lambda: xt.Cavity(voltage=3.0), 1e-14),
+    ('limitrect', lambda: xt.LimitRect(min_x=5), 1e-14),
+    ('limitellipse', lambda: xt.LimitEllipse(a=10, b=10), 1e-14),
+    ('quadrupole', lambda: xt.Quadrupole(length=1.0, k1=0.3), 5e-14),
+    ('sextupole', lambda: xt.Sextupole(length=0.5, k2=0.4), 5e-14),
+    ('octupole', lambda: xt.Octupole(length=0.5, k3=2.0), 5e-14),
+    ('bend', lambda: xt.Bend(length=2.0, angle=0.05), 5e-14),
+    ('bend_k1', lambda: xt.Bend(length=2.0, angle=0.05, k1=0.01, model='mat-kick-mat'), 5e-14),
+    ('bend_edges', lambda: xt.Bend(length=2.0, angle=0.05, edge_entry_angle=0.01,
+                                   edge_exit_angle=0.02), 5e-14),
+]
+
+
+@pytest.mark.parametrize('name,mk,tol', _BT_ELEMENTS, ids=[e[0] for e in _BT_ELEMENTS])
+def test_backtrack_round_trip(name, mk, tol):
+    """Forward then backward through one element returns the particle to
+    its start (the reference's round-trip tests: 1e-14, magnets x 5e-14)."""
+    line = xt.Line(elements=[mk()])
+    line.build_tracker(backend=OracleBackend())
+    p = xt.Particles(p0c=25.92e9, x=1e-3, px=1e-5, y=-2e-3, py=-1.5e-5, delta=1e-2,
+                     zeta=1.0)
+    p0 = p.copy()
+    line.track(p)
+    line.track(p, backtrack=True)
+    a, b = p.to_numpy(), p0.to_numpy()
+    for k in ('x', 'px', 'y', 'py', 'zeta', 'delta'):
+        assert abs(a[k][0] - b[k][0]) <= tol * max(1.0, abs(b[k][0])), (name, k)
+    assert abs(a['s'][0]) < 1e-7
+    if name != 'limitrect':
+        assert a['at_element'][0] == 0 and a['state'][0] == 1
+
+
+def test_backtrack_element_by_element_matches_forward():
+    """ONE_TURN_EBE records of a forward turn and of the backtracked turn
+    coincide element by element (xtrack tests/test_tracker.py:528-553,
+    1e-10); the particle returns to its start."""
+    from helpers import mixed_line
+    line = mixed_line(n_cells=2, aperture=0.5, with_cavity=False)
+    line.build_tracker(backend=OracleBackend())
+    p = xt.Particles(p0c=7e12, x=1e-4, px=1e-6, y=2e-4, py=3e-6, zeta=0.01, delta=1e-4)
+    p0 = p.copy()
+    line.track(p, turn_by_turn_monitor='ONE_TURN_EBE')
+    fwd = line.record_last_track
+    line.track(p, backtrack=True, turn_by_turn_monitor='ONE_TURN_EBE')
+    bwd = line.record_last_track
+    for k in ('x', 'px', 'y', 'py', 'zeta', 'delta'):
+        assert np.allclose(getattr(fwd, k), getattr(bwd, k), rtol=0, atol=1e-10), k
+    a, b = p.to_numpy(), p0.to_numpy()
+    for k in ('x', 'px', 'y', 'py', 'zeta', 'delta'):
+        assert abs(a[k][0] - b[k][0]) < 1e-13, k
+    assert a['at_turn'][0] == 0 and a['at_element'][0] == 0
+
+
+def test_backtrack_bookkeeping():
+    """tracker.py:629-642, 699-731: at_turn decreases by one per backtracked
+    turn, at_element is the number of elements at the turn start and s the
+    line length, and at_element counts down; a partial backward range."""
+    line = xt.Line(elements=[xt.Drift(length=1.0) for _ in range(6)])
+    line.build_tracker(backend=OracleBackend())
+    p = xt.Particles(p0c=7e12, x=[1e-3, 2e-3], px=[1e-5, -1e-5])
+    p0 = p.copy()
+    line.track(p, num_turns=3)
+    h = p.to_numpy()
+    assert list(h['at_turn']) == [3, 3] and list(h['at_element']) == [0, 0]
+    line.track(p, num_turns=3, backtrack=True)
+    h = p.to_numpy()
+    assert list(h['at_turn']) == [0, 0] and list(h['at_element']) == [0, 0]
+    assert np.allclose(h['x'], p0.to_numpy()['x'], rtol=0, atol=1e-16)
+    q = p0.copy()
+    line.track(q, ele_start=2, ele_stop=5, backtrack=True)
+    h = q.to_numpy()
+    # elements 4, 3, 2 backwards, starting from at_element 0
+    assert list(h['at_element']) == [-3, -3] and list(h['at_turn']) == [0, 0]
+    assert np.allclose(h['s'], -3.0)

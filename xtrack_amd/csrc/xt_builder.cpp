@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/xtrack_amd.h"
@@ -123,6 +124,7 @@ void polar_consts(double h, double s, double *ca, double *sa, double *t2, double
 struct Builder {
     std::vector<double> &prog;
     std::string &err;
+    bool bt = false;   // build the backtracking program (XS_FLAG_BACKTRACK)
     Builder(std::vector<double> &p, std::string &e) : prog(p), err(e) {}
 
     // MAGNET_DRIFT / RF_DRIFT (track_magnet_drift.h:468-555)
@@ -376,13 +378,24 @@ struct Builder {
             err = "not a magnet";
             return false;
         }
-        // forward tracking (track_magnet.h:444-450), weight = 1
+        // forward tracking or backtracking (track_magnet.h:413-443), weight = 1:
+        // backtracking negates the body length, the body factor and hxl,
+        // swaps the entry and exit edges and flips the linear edge kicks
         const double weight = 1.0;
         const double factor_knl_ksl = 1.0;
-        const double core_length = m.length * weight;
-        const double factor_body = factor_knl_ksl * weight;
+        const double core_length = bt ? -m.length * weight : m.length * weight;
+        const double factor_body = bt ? -factor_knl_ksl * weight : factor_knl_ksl * weight;
         const double factor_edge = factor_knl_ksl;
-        const double fbe = 1.;
+        const double fbe = bt ? -1. : 1.;
+        if (bt) {
+            m.hxl = -m.hxl;
+            std::swap(m.edge_entry_active, m.edge_exit_active);
+            std::swap(m.edge_entry_model, m.edge_exit_model);
+            std::swap(m.edge_entry_angle, m.edge_exit_angle);
+            std::swap(m.edge_entry_fdown, m.edge_exit_fdown);
+            std::swap(m.edge_entry_fint, m.edge_exit_fint);
+            std::swap(m.edge_entry_hgap, m.edge_exit_hgap);
+        }
 
         if (m.edge_entry_active &&
             !edge(m, false, m.edge_entry_model, m.edge_entry_angle, m.edge_entry_fdown,
@@ -534,7 +547,9 @@ struct Builder {
         int64_t model = (int64_t)p[XTM_C_MODEL];
         int64_t integrator = (int64_t)p[XTM_C_INTEGRATOR];
         const double weight = 1.;
-        const double body_length = length;
+        // backtracking (track_rf.h:364-373): negative body length and voltage
+        const double body_length = bt ? -length : length;
+        if (bt) voltage = -voltage;
         if (integrator == 0) integrator = 3;
         if (model == 0) model = 6;
         if (model == -1) { integrator = 3; num_kicks = 1; }
@@ -614,10 +629,12 @@ struct Builder {
         switch (type) {
         case XTM_MARKER: break;
         case XTM_DRIFT: {
+            // drift.h:14-40 (backtracking: negative length)
             int64_t model = (int64_t)p[XTM_D_MODEL];
             if (model == 0) model = 1;
-            if (model == 1) { P.op(OP_DRIFT_EXP); P.arg(p[XTM_D_LENGTH]); }
-            else if (model == 2) { P.op(OP_DRIFT_EXACT); P.arg(p[XTM_D_LENGTH]); }
+            const double dl = bt ? -p[XTM_D_LENGTH] : p[XTM_D_LENGTH];
+            if (model == 1) { P.op(OP_DRIFT_EXP); P.arg(dl); }
+            else if (model == 2) { P.op(OP_DRIFT_EXACT); P.arg(dl); }
             else { err = "drift model not supported"; ok = false; }
             P.op(OP_GLOBAL_APER);
             break; }
@@ -673,12 +690,17 @@ size_t raw_size(int type, const double *p) {
 
 int xt_build_program(const int32_t *types, const int64_t *offsets, const double *params,
                      int64_t n, std::vector<int64_t> &entry, std::vector<double> &prog,
-                     std::string &err) {
+                     std::string &err, std::vector<int64_t> *entry_bt) {
     entry.assign(n, 0);
     prog.clear();
     prog.reserve(1024);
     Builder B(prog, err);
-    std::map<std::string, int64_t> seen;   // raw record bytes -> program offset
+    // (direction, raw record bytes) -> program offset
+    std::map<std::string, int64_t> seen;
+    for (int dir = 0; dir < (entry_bt ? 2 : 1); dir++) {
+    std::vector<int64_t> &ent = dir == 0 ? entry : *entry_bt;
+    if (dir == 1) ent.assign(n, 0);
+    B.bt = dir == 1;
     for (int64_t i = 0; i < n; i++) {
         const int type = types[i];
         if (type < 0 || type >= XTM_NUM_TYPES) {
@@ -693,17 +715,19 @@ int xt_build_program(const int32_t *types, const int64_t *offsets, const double 
             }
         }
         const size_t sz = raw_size(type, p);
-        std::string key(reinterpret_cast<const char *>(&type), sizeof(type));
+        std::string key(1, (char)dir);
+        key.append(reinterpret_cast<const char *>(&type), sizeof(type));
         key.append(reinterpret_cast<const char *>(p), sz * sizeof(double));
         auto it = seen.find(key);
-        if (it != seen.end()) { entry[i] = it->second; continue; }
+        if (it != seen.end()) { ent[i] = it->second; continue; }
         int64_t off = B.element(type, p);
         if (off < 0) {
-            err = "element " + std::to_string(i) + ": " + err;
+            err = "element " + std::to_string(i) + (dir ? " (backtracking)" : "") + ": " + err;
             return XTM_ERR_UNSUPPORTED;
         }
         seen.emplace(std::move(key), off);
-        entry[i] = off;
+        ent[i] = off;
+    }
     }
     if (prog.empty()) prog.push_back(ibits(OP_END));
     return XTM_OK;

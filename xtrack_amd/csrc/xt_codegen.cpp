@@ -195,14 +195,21 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
     s += "struct Runner {\n"
          "    const int32_t *__restrict__ rec;\n"
          "    const int64_t *__restrict__ entry;\n"
-         "    const double *__restrict__ prog;\n"
-         "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, int64_t e,\n"
-         "                          const xt::TrackCtx &c) const {\n";
+         "    const double *__restrict__ prog;\n";
     if (loads)
-        s += "        const double *__restrict__ rp = prog + entry[e];\n";
+        s += "    struct Key { int32_t r; int64_t pc; };\n"
+             "    XT_HD Key key(int64_t e) const { return Key{rec[e], entry[e]}; }\n"
+             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
+             "                          const xt::TrackCtx &c) const {\n"
+             "        const double *__restrict__ rp = prog + k.pc;\n"
+             "        switch (k.r) {\n";
     else
-        s += "        const double *__restrict__ rp = prog;\n";
-    s += "        switch (rec[e]) {\n";
+        s += "    using Key = int32_t;\n"
+             "    XT_HD Key key(int64_t e) const { return rec[e]; }\n"
+             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key r,\n"
+             "                          const xt::TrackCtx &c) const {\n"
+             "        const double *__restrict__ rp = prog;\n"
+             "        switch (r) {\n";
     for (int32_t r = 0; r < k; r++)
         s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
              "(p, m, prog, rp, c); break;\n";
@@ -226,7 +233,7 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
          "    // generated code for chi == 1 particles, the interpreter for the others\n"
-         "    xt::track_particle_with(P, i, xt_gen::Runner{rec, entry, prog},\n"
+         "    xt::track_particle_with<false>(P, i, xt_gen::Runner{rec, entry, prog},\n"
          "                            xt::ProgramRunner{entry, prog},\n"
          "                            c, num_turns, ele_start, num_ele_track,\n"
          "                            flag_end_turn_actions, flag_reset_s, flag_monitor, M);\n"
@@ -240,7 +247,7 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    xtm_monitor m0 = {};\n"
          "    const xt::TrackCtx c{flags, line_length, P->q0, P->t_sim};\n"
          "    for (int64_t i = 0; i < P->capacity; i++)\n"
-         "        xt::track_particle_with(*P, i, xt_gen::Runner{rec, entry, prog},\n"
+         "        xt::track_particle_with<false>(*P, i, xt_gen::Runner{rec, entry, prog},\n"
          "                                xt::ProgramRunner{entry, prog}, c, num_turns,\n"
          "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
          "                                flag_reset_s, flag_monitor, M ? *M : m0);\n"

@@ -56,13 +56,14 @@ constexpr int BLOCK = 256;
 
 __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     const int64_t *__restrict__ entry, const double *__restrict__ prog, double line_length,
-    xtm_particles P, int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
-    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor, xtm_monitor M,
-    uint64_t flags) {
+    int64_t num_ele_line, xtm_particles P, int32_t num_turns, int32_t ele_start,
+    int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,
+    int32_t flag_monitor, xtm_monitor M, uint64_t flags) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= P.capacity) return;
-    xt::track_particle(P, i, entry, prog, line_length, num_turns, ele_start, num_ele_track,
-                       flag_end_turn_actions, flag_reset_s, flag_monitor, M, flags);
+    xt::track_particle(P, i, entry, prog, line_length, num_ele_line, num_turns, ele_start,
+                       num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor, M,
+                       flags);
 }
 
 __global__ void math_kernel(int32_t fn, const double *__restrict__ x, double *__restrict__ y,
@@ -105,9 +106,11 @@ struct xtm_line {
     int64_t num_elements;
     int64_t num_records;
     int64_t *d_entry;
+    int64_t *d_entry_bt;   // backtracking programs (XS_FLAG_BACKTRACK)
     double *d_prog;
     double line_length;
-    std::vector<int64_t> entry;   // host copies (operation counting)
+    std::vector<int64_t> entry;   // host copies (operation counting, code generation)
+    std::vector<int64_t> entry_bt;
     std::vector<double> prog;
     // generated line code (xtm_line_specialize)
     int32_t *d_rec = nullptr;
@@ -200,10 +203,10 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
     if (!out || num_elements < 0 || (num_elements > 0 && (!types || !offsets || !params)))
         return fail(XTM_ERR_ARG, "xtm_line_create: invalid arguments");
     *out = nullptr;
-    std::vector<int64_t> entry;
+    std::vector<int64_t> entry, entry_bt;
     std::vector<double> prog;
     std::string err;
-    int st = xt_build_program(types, offsets, params, num_elements, entry, prog, err);
+    int st = xt_build_program(types, offsets, params, num_elements, entry, prog, err, &entry_bt);
     if (st != XTM_OK) return fail(st, err);
     xtm_line *L = new xtm_line();
     L->device = device;
@@ -211,9 +214,11 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
     L->num_records = xt_count_records(entry);
     L->line_length = line_length;
     L->d_entry = nullptr;
+    L->d_entry_bt = nullptr;
     L->d_prog = nullptr;
     if (device < 0) {   // host-only handle: inspection (records, op counts), no tracking
         L->entry = std::move(entry);
+        L->entry_bt = std::move(entry_bt);
         L->prog = std::move(prog);
         *out = L;
         return XTM_OK;
@@ -225,13 +230,16 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
     }
     const size_t ne = entry.empty() ? 1 : entry.size();
     if ((e = hipMalloc(&L->d_entry, ne * sizeof(int64_t))) != hipSuccess ||
+        (e = hipMalloc(&L->d_entry_bt, ne * sizeof(int64_t))) != hipSuccess ||
         (e = hipMalloc(&L->d_prog, prog.size() * sizeof(double))) != hipSuccess) {
         xtm_line_destroy(L);
         return hip_fail(e, "hipMalloc");
     }
     if (!entry.empty() &&
-        (e = hipMemcpy(L->d_entry, entry.data(), entry.size() * sizeof(int64_t),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
+        ((e = hipMemcpy(L->d_entry, entry.data(), entry.size() * sizeof(int64_t),
+                        hipMemcpyHostToDevice)) != hipSuccess ||
+         (e = hipMemcpy(L->d_entry_bt, entry_bt.data(), entry_bt.size() * sizeof(int64_t),
+                        hipMemcpyHostToDevice)) != hipSuccess)) {
         xtm_line_destroy(L);
         return hip_fail(e, "hipMemcpy");
     }
@@ -241,6 +249,7 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
         return hip_fail(e, "hipMemcpy");
     }
     L->entry = std::move(entry);
+    L->entry_bt = std::move(entry_bt);
     L->prog = std::move(prog);
     *out = L;
     return XTM_OK;
@@ -250,6 +259,7 @@ int32_t xtm_line_destroy(xtm_line *L) {
     if (!L) return XTM_OK;
     if (L->device >= 0) (void)hipSetDevice(L->device);
     if (L->d_entry) (void)hipFree(L->d_entry);
+    if (L->d_entry_bt) (void)hipFree(L->d_entry_bt);
     if (L->d_prog) (void)hipFree(L->d_prog);
     if (L->d_rec) (void)hipFree(L->d_rec);
     if (L->module) (void)hipModuleUnload(L->module);
@@ -351,8 +361,6 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
         return fail(XTM_ERR_ARG, "xtm_track_line: invalid monitor arguments");
     if (flag_monitor && monitor->n_repetitions > 1 && monitor->repetition_period <= 0)
         return fail(XTM_ERR_ARG, "xtm_track_line: invalid monitor repetition period");
-    if ((track_flags >> XTM_FLAG_BACKTRACK) & 1)
-        return fail(XTM_ERR_UNSUPPORTED, "backtracking is not supported");
     if ((track_flags >> XTM_FLAG_SR_TAPER) & 1)
         return fail(XTM_ERR_UNSUPPORTED, "SR tapering is not supported");
     if (P->capacity < 0) return fail(XTM_ERR_ARG, "xtm_track_line: negative capacity");
@@ -365,7 +373,8 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
     else std::memset(&M, 0, sizeof(M));
     const int64_t nblocks = (P->capacity + BLOCK - 1) / BLOCK;
     if (nblocks > 0x7fffffff) return fail(XTM_ERR_ARG, "too many particles");
-    if (L->kernel) {
+    const bool backtrack = (track_flags >> XTM_FLAG_BACKTRACK) & 1;
+    if (L->kernel && !backtrack) {
         xtm_particles Pv = *P;
         const int32_t *rec = L->d_rec;
         const int64_t *entry = L->d_entry;
@@ -382,8 +391,11 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
         if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
         return XTM_OK;
     }
+    /* backtracking always runs on the interpreter, with the backtracking
+       programs */
     hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
-                       (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length, *P,
+                       (hipStream_t)stream, backtrack ? L->d_entry_bt : L->d_entry, L->d_prog,
+                       L->line_length, (int64_t)L->num_elements, *P,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
                        flag_reset_s_at_end_turn, flag_monitor, M, track_flags);
     e = hipGetLastError();

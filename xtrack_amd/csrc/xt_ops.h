@@ -779,6 +779,8 @@ XT_HD void op_global_aper(PState<R> &p, const bool ignore_global) {
 struct TrackCtx {
     uint64_t flags;
     double line_length, q0, t_sim;
+    int64_t num_ele_line;   // elements in the line (backtracking turn start)
+    XT_HD bool backtrack() const { return flags & 1; }              // XS_FLAG_BACKTRACK
     XT_HD bool ignore_local() const { return (flags >> 4) & 1; }   // XS_FLAG_IGNORE_LOCAL_APERTURE
     XT_HD bool ignore_global() const { return (flags >> 3) & 1; }  // XS_FLAG_IGNORE_GLOBAL_APERTURE
     XT_HD bool kill_cavity() const { return (flags >> 2) & 1; }    // XS_FLAG_KILL_CAVITY_KICK
@@ -901,7 +903,11 @@ XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i
 XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int64_t i,
                                double x, double px, double y, double py, double zeta,
                                double delta, double rpp, double rvv, double s, double chi,
-                               int64_t at_turn, int64_t state, int64_t at_element) {
+                               int64_t at_turn, int64_t state, int64_t at_element,
+                               bool backtrack) {
+    /* particles_monitor.h: a turn-by-turn record is not taken while
+       backtracking (element-by-element records are) */
+    if (backtrack && !M.ebe_mode) return;
     PState<double> p;
     p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.delta = delta;
     p.rpp = rpp; p.rvv = rvv; p.s = s; p.chi = chi; p.at_turn = at_turn; p.state = state;
@@ -930,7 +936,7 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
 
 #define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
     monitor_record(M, P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta, p.rpp, p.rvv, p.s,      \
-                   p.chi, p.at_turn, p.state, (AT_ELEMENT))
+                   p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
 
 /* ---- one particle through [ele_start, ele_start+num_ele_track) for
    num_turns turns: the per-thread body of track_line (tracker.py:596-734)
@@ -939,13 +945,13 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
    at_element: the reference adds 1 after every element the particle
    survives and sets 0 at the end of a turn; here at_element = base + e at
    element e, with base updated at the end of each turn.
-   run(p, m, e, c) tracks element e: the program interpreter, or the
-   line's generated code.
+   run(p, m, run.key(e), c) tracks element e: the program interpreter, or
+   the line's generated code.
    CHI1: the particle's chi is exactly 1.0 and the element code sees the
    constant, so products with chi fold away (chi * k == k exactly) instead
    of being evaluated, or hoisted out of the loops as per-particle values
    that occupy registers for the whole launch. */
-template <bool CHI1, class Runner>
+template <bool CHI1, bool BT, class Runner>
 XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &run,
                                const TrackCtx &c, int32_t num_turns, int32_t ele_start,
                                int32_t num_ele_track, int32_t flag_end_turn_actions,
@@ -960,29 +966,65 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
     refresh_flags(p);
     const PRef m{&P, i};
     const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
-    int64_t base = P.at_element[i] - ele_start;
     int64_t e_lost = -1;
-    for (int32_t iturn = 0; iturn < num_turns; iturn++) {
-        if (flag_monitor == 1) XT_MONITOR_RECORD(base + ele_start);
-        for (int64_t e = ele_start; e < ele_stop; e++) {
-            if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
-            run(p, m, e, c);
-            if (p.state <= 0) { e_lost = e; break; }
+    int64_t at_element;
+    if (!BT) {
+        int64_t base = P.at_element[i] - ele_start;
+        for (int32_t iturn = 0; iturn < num_turns; iturn++) {
+            if (flag_monitor == 1) XT_MONITOR_RECORD(base + ele_start);
+            /* the next element's key (record index / program offset) is
+               loaded one element ahead, so its scalar load overlaps the
+               current element's arithmetic */
+            typename Runner::Key key{};
+            if (ele_start < ele_stop) key = run.key(ele_start);
+            for (int64_t e = ele_start; e < ele_stop; e++) {
+                const typename Runner::Key next = run.key(e + 1 < ele_stop ? e + 1 : e);
+                if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
+                run(p, m, key, c);
+                if (p.state <= 0) { e_lost = e; break; }
+                key = next;
+            }
+            if (e_lost >= 0) break;
+            if (flag_monitor == 2) XT_MONITOR_RECORD(base + ele_stop);
+            if (flag_end_turn_actions > 0) {
+                p.at_turn += 1;
+                base = -(int64_t)ele_start;   /* at_element = 0 at the next turn's start */
+                if (flag_reset_s > 0) p.s = 0.;
+            } else {
+                base += num_ele_track;
+            }
         }
-        if (e_lost >= 0) break;
-        if (flag_monitor == 2) XT_MONITOR_RECORD(base + ele_stop);
-        if (flag_end_turn_actions > 0) {
-            p.at_turn += 1;
-            base = -(int64_t)ele_start;   /* at_element = 0 at the next turn's start */
-            if (flag_reset_s > 0) p.s = 0.;
-        } else {
-            base += num_ele_track;
+        at_element = base + (e_lost >= 0 ? e_lost : (int64_t)ele_start);
+    } else {
+        /* backtracking (tracker.py:629-642, 699-731): elements in reverse
+           order, at_element decreasing, the previous turn entered before
+           the element loop (at_turn - 1, at_element = number of elements,
+           s = line length); at element e, at_element = base + e */
+        int64_t at_el = P.at_element[i];
+        int64_t base = 0;
+        for (int32_t iturn = 0; iturn < num_turns; iturn++) {
+            if (flag_end_turn_actions > 0) {
+                p.at_turn -= 1;
+                at_el = c.num_ele_line;
+                if (flag_reset_s > 0) p.s = c.line_length;
+            }
+            base = at_el - (ele_stop - 1);
+            for (int64_t e = ele_stop - 1; e >= ele_start; e--) {
+                if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
+                run(p, m, run.key(e), c);
+                if (p.state <= 0) { e_lost = e; break; }
+            }
+            if (e_lost >= 0) break;
+            at_el = base + ele_start - 1;
+            if (flag_monitor == 2) XT_MONITOR_RECORD(at_el);
+            if (flag_monitor == 1) XT_MONITOR_RECORD(at_el);
         }
+        at_element = e_lost >= 0 ? base + e_lost : at_el;
     }
     P.x[i] = p.x; P.px[i] = p.px; P.y[i] = p.y; P.py[i] = p.py;
     P.zeta[i] = p.zeta; P.delta[i] = p.delta;
     P.rpp[i] = p.rpp; P.rvv[i] = p.rvv; P.s[i] = p.s;
-    P.at_element[i] = base + (e_lost >= 0 ? e_lost : (int64_t)ele_start);
+    P.at_element[i] = at_element;
     P.at_turn[i] = p.at_turn; P.state[i] = p.state;
 }
 
@@ -990,37 +1032,50 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
 struct ProgramRunner {
     const int64_t *__restrict__ entry;
     const double *__restrict__ prog;
-    XT_HD void operator()(PState<double> &p, const PRef &m, int64_t e, const TrackCtx &c) const {
-        run_element(p, m, prog, entry[e], c);
+    using Key = int64_t;
+    XT_HD Key key(int64_t e) const { return entry[e]; }
+    XT_HD void operator()(PState<double> &p, const PRef &m, Key pc, const TrackCtx &c) const {
+        run_element(p, m, prog, pc, c);
     }
 };
 
 /* run_chi1 serves particles with chi == 1.0 (track_particle_loop<true>),
-   run_other all others */
-template <class Runner1, class RunnerOther>
+   run_other all others.  With ALLOW_BT, c.backtrack() selects the
+   backtracking loop (run_other, which must then execute the backtracking
+   programs); without, the caller guarantees forward tracking. */
+template <bool ALLOW_BT, class Runner1, class RunnerOther>
 XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner1 &run_chi1,
                                const RunnerOther &run_other, const TrackCtx &c,
                                int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
                                int32_t flag_end_turn_actions, int32_t flag_reset_s,
                                int32_t flag_monitor, const xtm_monitor &M) {
     if (!(P.state[i] > 0)) return;
-    if (P.chi[i] == 1.0)
-        track_particle_loop<true>(P, i, run_chi1, c, num_turns, ele_start, num_ele_track,
-                                  flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+    if (ALLOW_BT && c.backtrack())
+        track_particle_loop<false, true>(P, i, run_other, c, num_turns, ele_start,
+                                         num_ele_track, flag_end_turn_actions, flag_reset_s,
+                                         flag_monitor, M);
+    else if (P.chi[i] == 1.0)
+        track_particle_loop<true, false>(P, i, run_chi1, c, num_turns, ele_start,
+                                         num_ele_track, flag_end_turn_actions, flag_reset_s,
+                                         flag_monitor, M);
     else
-        track_particle_loop<false>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
-                                   flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+        track_particle_loop<false, false>(P, i, run_other, c, num_turns, ele_start,
+                                          num_ele_track, flag_end_turn_actions, flag_reset_s,
+                                          flag_monitor, M);
 }
 
+/* entry: the forward or, when flags hold XS_FLAG_BACKTRACK, the
+   backtracking programs; num_ele_line: elements in the line */
 XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__restrict__ entry,
                           const double *__restrict__ prog, double line_length,
-                          int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
-                          int32_t flag_end_turn_actions, int32_t flag_reset_s,
-                          int32_t flag_monitor, const xtm_monitor &M, uint64_t flags) {
-    const TrackCtx c{flags, line_length, P.q0, P.t_sim};
+                          int64_t num_ele_line, int32_t num_turns, int32_t ele_start,
+                          int32_t num_ele_track, int32_t flag_end_turn_actions,
+                          int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor &M,
+                          uint64_t flags) {
+    const TrackCtx c{flags, line_length, P.q0, P.t_sim, num_ele_line};
     const ProgramRunner run{entry, prog};
-    track_particle_with(P, i, run, run, c, num_turns, ele_start, num_ele_track,
-                        flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+    track_particle_with<true>(P, i, run, run, c, num_turns, ele_start, num_ele_track,
+                              flag_end_turn_actions, flag_reset_s, flag_monitor, M);
 }
 
 #undef XT_MONITOR_RECORD

@@ -185,12 +185,17 @@ class Tracker:
     def _track(self, particles, ele_start=0, ele_stop=None, num_elements=None,
                num_turns=None, turn_by_turn_monitor=None, freeze_longitudinal=False,
                backtrack=False, time=False, with_progress=False):
-        if backtrack:
-            raise NotImplementedError('backtracking is not supported by the HIP tracker')
+        if backtrack is not False and backtrack is not True and backtrack != 'force':
+            raise ValueError("backtrack must be True, False or 'force'")
         if freeze_longitudinal:
             raise NotImplementedError('freeze_longitudinal is not supported by the HIP tracker')
         t0 = perf_counter() if time else None
         particles, restore = self._place(particles)
+        # tracker.py:1222-1236: backtracking sets XS_FLAG_BACKTRACK for this
+        # call (every element of this build has a backtracking map)
+        old_bt = self.track_flags.XS_FLAG_BACKTRACK
+        if backtrack is not False:
+            self.track_flags.XS_FLAG_BACKTRACK = True
         try:
             if with_progress:
                 self._track_in_batches(particles, with_progress, ele_start, ele_stop,
@@ -202,6 +207,7 @@ class Tracker:
             if time:
                 self.backend.synchronize()
         finally:
+            self.track_flags.XS_FLAG_BACKTRACK = old_bt
             if restore is not None:
                 restore()
         self.time_last_track = (perf_counter() - t0) if time else None
