@@ -45,7 +45,9 @@ extern "C" {
 #define XTM_CAVITY 7
 #define XTM_LIMITRECT 8
 #define XTM_LIMITELLIPSE 9
-#define XTM_NUM_TYPES 10
+#define XTM_DIPOLEEDGE 10
+#define XTM_XYSHIFTED 11
+#define XTM_NUM_TYPES 12
 
 /* ---- raw parameter layout ---------------------------------------------------
  * Element i occupies params[offsets[i] ...].  Integer fields are stored as
@@ -122,6 +124,26 @@ extern "C" {
 #define XTM_LE_B_SQU 1
 #define XTM_LE_A_B_SQU 2
 #define XTM_LE_SIZE 3
+
+/* DIPOLEEDGE (dipole_edge.py:38-49): r21, r43 as the class derives them
+ * from k, e1, e1_fd, hgap, fint; model 0 linear, 1 full, -1 suppressed */
+#define XTM_DE_R21 0
+#define XTM_DE_R43 1
+#define XTM_DE_MODEL 2
+#define XTM_DE_DELTA_TAPER 3
+#define XTM_DE_SIZE 4
+
+/* XYSHIFTED: an element displaced transversely (shift_x, shift_y; every
+ * other misalignment field zero), wrapped as in
+ * track_local_particle_with_transformations.h with the straight-frame
+ * transforms of track_misalignments.h:38-110.  Record: shift_x, shift_y,
+ * inner type, then the inner element's record.  Inner types: Multipole,
+ * Quadrupole, Sextupole, Octupole, Cavity, LimitRect, LimitEllipse,
+ * DipoleEdge. */
+#define XTM_XS_SHIFT_X 0
+#define XTM_XS_SHIFT_Y 1
+#define XTM_XS_INNER_TYPE 2
+#define XTM_XS_INNER 3
 
 /* ---- track flags (xtrack/track_flags.py:5-12) ------------------------------ */
 #define XTM_FLAG_BACKTRACK 0

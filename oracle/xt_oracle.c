@@ -914,6 +914,33 @@ static void limitellipse_track(LP *part0, const double *p) {
     } END_FOR_ACTIVE
 }
 
+/* dipoleedge.h:17-50, linear model: r21, r43 arrive as the element class
+   derived them (dipole_edge.py:123-131) */
+static int dipoleedge_track(LP *part0, const double *p) {
+    int64_t const model = (int64_t)p[XTM_DE_MODEL];
+    if (model == 0) {
+        double const delta_taper = p[XTM_DE_DELTA_TAPER];
+        double r21 = p[XTM_DE_R21];
+        double r43 = p[XTM_DE_R43];
+        r21 = r21 * (1 + delta_taper);
+        r43 = r43 * (1 + delta_taper);
+        if (check_flag(part0, XTM_FLAG_BACKTRACK)) {
+            r21 = -r21;
+            r43 = -r43;
+        }
+        FOR_ACTIVE(part0, part) {          /* track_dipole_edge_linear.h */
+            double const x = GET(x);
+            double const y = GET(y);
+            double const chi = GET(chi);
+            ADD(px, chi * r21 * x);
+            ADD(py, chi * r43 * y);
+        } END_FOR_ACTIVE
+        return XTM_OK;
+    }
+    if (model == -1) return XTM_OK;
+    return XTM_ERR_UNSUPPORTED;
+}
+
 /* drift.h:14-40 */
 static void drift_track(LP *part0, const double *p) {
     double length = p[XTM_D_LENGTH];
@@ -986,7 +1013,54 @@ static int is_static_thick(int type) {
            type == XTM_SEXTUPOLE || type == XTM_OCTUPOLE || type == XTM_CAVITY;
 }
 
-static int track_element(LP *lp, int type, const double *p) {
+static int track_element_body(LP *lp, int type, const double *p);
+
+/* track_local_particle_with_transformations.h:80-132 with the straight
+   transforms of track_misalignments.h:38-110, for an element displaced by
+   shift_x, shift_y only (anchor, rotations and shift_s zero) */
+static int xyshifted_track(LP *part0, const double *p) {
+    double const dx = p[XTM_XS_SHIFT_X], dy = p[XTM_XS_SHIFT_Y];
+    int const inner = (int)p[XTM_XS_INNER_TYPE];
+    const double *q = p + XTM_XS_INNER;
+    double length = 0.0;
+    if (inner == XTM_MULTIPOLE) length = q[XTM_M_ISTHICK] > 0 ? q[XTM_M_LENGTH] : 0.0;
+    else if (inner == XTM_QUADRUPOLE || inner == XTM_SEXTUPOLE || inner == XTM_OCTUPOLE)
+        length = q[XTM_M_LENGTH];
+    else if (inner == XTM_CAVITY) length = q[XTM_C_LENGTH];
+    else if (!(inner == XTM_LIMITRECT || inner == XTM_LIMITELLIPSE || inner == XTM_DIPOLEEDGE))
+        return XTM_ERR_UNSUPPORTED;
+    double const anchor = 0.0, theta = 0.0, phi = 0.0;
+    double const mis_x_in = dx - anchor * M_COS(phi) * M_SIN(theta);
+    double const mis_y_in = dy - anchor * M_SIN(phi);
+    double const neg_part_length = anchor - length;
+    double const mis_x_out = neg_part_length * M_COS(phi) * M_SIN(theta) - dx;
+    double const mis_y_out = neg_part_length * M_SIN(phi) - dy;
+    int const backtrack = check_flag(part0, XTM_FLAG_BACKTRACK);
+    /* entry (exit with backtrack): XY_SHIFT adds -dx', -dy' */
+    double const pre_x = backtrack ? -(-mis_x_out) : -mis_x_in;
+    double const pre_y = backtrack ? -(-mis_y_out) : -mis_y_in;
+    double const post_x = backtrack ? -(-mis_x_in) : -mis_x_out;
+    double const post_y = backtrack ? -(-mis_y_in) : -mis_y_out;
+    /* The serial CPU and GPU contexts run every block of an element over
+     * the particles active when the element starts (track.h:20-31, 58-62;
+     * the active range shrinks only between elements, tracker.py:702):
+     * particles lost inside the element are shifted back too. */
+    int64_t const n = part0->endpart - part0->ipart;
+    int64_t *active = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!active) return XTM_ERR_NOMEM;
+    int64_t na = 0;
+    FOR_ACTIVE(part0, part) { active[na++] = part->ipart; ADD(x, pre_x); ADD(y, pre_y); }
+    END_FOR_ACTIVE
+    int st = track_element_body(part0, inner, q);
+    for (int64_t k = 0; k < na; k++) {
+        LP lp = *part0; LP *part = &lp; part->ipart = active[k];
+        ADD(x, post_x); ADD(y, post_y);
+    }
+    free(active);
+    return st;
+}
+
+static int track_element_body(LP *lp, int type, const double *p) {
     int st = XTM_OK;
     switch (type) {
     case XTM_MARKER: break;
@@ -996,10 +1070,19 @@ static int track_element(LP *lp, int type, const double *p) {
     case XTM_CAVITY: st = cavity_track(lp, p); break;
     case XTM_LIMITRECT: limitrect_track(lp, p); break;
     case XTM_LIMITELLIPSE: limitellipse_track(lp, p); break;
+    case XTM_DIPOLEEDGE: st = dipoleedge_track(lp, p); break;
+    case XTM_XYSHIFTED: st = xyshifted_track(lp, p); break;
     default: return XTM_ERR_ARG;
     }
+    return st;
+}
+
+static int track_element(LP *lp, int type, const double *p) {
+    int st = track_element_body(lp, type, p);
     if (st) return st;
-    if (is_static_thick(type)) global_aperture_check(lp);
+    /* the class's static thickness decides the global check (tracker.py:681) */
+    int const cls = type == XTM_XYSHIFTED ? (int)p[XTM_XS_INNER_TYPE] : type;
+    if (is_static_thick(cls)) global_aperture_check(lp);
     return XTM_OK;
 }
 

@@ -186,6 +186,9 @@ class GeneratedHostBackend(OracleBackend):
                    flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor,
                    monitor, track_flags):
         glib, rec, entry, prog, line_length = handle
+        if track_flags & 1:
+            # the library runs backtracking on the interpreter
+            raise NotImplementedError('generated line code tracks forward only')
         pst = _native.make_particles_struct(particles)
         mst = _native.make_monitor_struct(monitor) if monitor is not None else None
         glib.xtm_line_host(rec.ctypes.data, entry.ctypes.data, prog.ctypes.data, line_length,

@@ -71,10 +71,19 @@ def test_builder_rejects_unsupported_models(el):
     assert b'element 1' in lib.xtm_last_error()
 
 
-def test_misalignment_rejected_at_table_build():
+def test_misalignment_table_build():
+    # transverse shifts wrap the element (track_misalignments.h:38-110)
     q = xt.Quadrupole(length=1.0, k1=0.1, shift_x=1e-3)
-    with pytest.raises(NotImplementedError):
-        build_raw_table([q])
+    types, offsets, params = build_raw_table([q])
+    assert list(types) == [xt.elements.T_XYSHIFTED]
+    assert list(params[:3]) == [1e-3, 0.0, float(xt.elements.T_QUADRUPOLE)]
+    # other misalignments, and shifts of classes without straight-frame
+    # transformations, are rejected
+    for ee in (xt.Quadrupole(length=1.0, k1=0.1, rot_s_rad=1e-3),
+               xt.Quadrupole(length=1.0, k1=0.1, shift_s=1e-3),
+               xt.Bend(length=1.0, angle=1e-3, shift_y=1e-3)):
+        with pytest.raises(NotImplementedError):
+            build_raw_table([ee])
 
 
 def test_track_flags_register():

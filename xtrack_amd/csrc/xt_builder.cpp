@@ -531,7 +531,7 @@ struct Builder {
             !edge(m, true, m.edge_exit_model, m.edge_exit_angle, m.edge_exit_fdown,
                   m.edge_exit_fint, m.edge_exit_hgap, factor_edge, fbe))
             return false;
-        if (static_thick) Prog(prog).op(OP_GLOBAL_APER);
+        if (static_thick) global_aper = true;
         return true;
     }
 
@@ -615,16 +615,50 @@ struct Builder {
             err = "integrator " + std::to_string(integrator) + " is not supported";
             return false;
         }
-        Prog(prog).op(OP_GLOBAL_APER);
+        global_aper = true;
         return true;
     }
 
-    // element -> ops; returns start offset or -1
-    int64_t element(int type, const double *p) {
-        const int64_t start = (int64_t)prog.size();
+    // track_misalignments.h:38-110 for a straight frame with only the
+    // transverse shifts non-zero (anchor, rotations, shift_s zero): the
+    // entry adds -mis_x, the exit -mis_x' (reversed when backtracking)
+    bool xy_shifted(const double *p) {
+        const double dx = p[XTM_XS_SHIFT_X], dy = p[XTM_XS_SHIFT_Y];
+        const int inner = (int)p[XTM_XS_INNER_TYPE];
+        const double *q = p + XTM_XS_INNER;
+        double length = 0.0;   // IS_THICK / IS_THICK_DYNAMIC elements
+        switch (inner) {
+        case XTM_MULTIPOLE: length = q[XTM_M_ISTHICK] > 0 ? q[XTM_M_LENGTH] : 0.0; break;
+        case XTM_QUADRUPOLE: case XTM_SEXTUPOLE: case XTM_OCTUPOLE:
+            length = q[XTM_M_LENGTH]; break;
+        case XTM_CAVITY: length = q[XTM_C_LENGTH]; break;
+        case XTM_LIMITRECT: case XTM_LIMITELLIPSE: case XTM_DIPOLEEDGE: break;
+        default:
+            err = "transverse shifts are not supported for element type " + std::to_string(inner);
+            return false;
+        }
+        const double anchor = 0.0, theta = 0.0, phi = 0.0, ds = 0.0;
+        const double mis_x_in = dx - anchor * xt_cos(phi) * xt_sin(theta);
+        const double mis_y_in = dy - anchor * xt_sin(phi);
+        const double neg_part_length = anchor - length;
+        const double mis_x_out = neg_part_length * xt_cos(phi) * xt_sin(theta) - dx;
+        const double mis_y_out = neg_part_length * xt_sin(phi) - dy;
+        const double mis_s_in = ds - anchor * (xt_cos(phi) * xt_cos(theta) - 1);
+        const double mis_s_out = neg_part_length * (xt_cos(phi) * xt_cos(theta) - 1) - ds;
+        if (mis_s_in != 0.0 || mis_s_out != 0.0) { err = "longitudinal shift"; return false; }
         Prog P(prog);
-        pending_desc = false;
-        pending_kicks.clear();
+        // XYShift_single_particle(dx', dy') adds -dx', -dy'
+        if (!bt) { P.op(OP_XY_SHIFT); P.arg(-mis_x_in); P.arg(-mis_y_in); }
+        else { P.op(OP_XY_SHIFT); P.arg(-(-mis_x_out)); P.arg(-(-mis_y_out)); }
+        if (!body(inner, q)) return false;
+        if (!bt) { P.op(OP_XY_SHIFT); P.arg(-mis_x_out); P.arg(-mis_y_out); }
+        else { P.op(OP_XY_SHIFT); P.arg(-(-mis_x_in)); P.arg(-(-mis_y_in)); }
+        return true;
+    }
+
+    // ops of one element without its global aperture check and OP_END
+    bool body(int type, const double *p) {
+        Prog P(prog);
         bool ok = true;
         switch (type) {
         case XTM_MARKER: break;
@@ -636,7 +670,7 @@ struct Builder {
             if (model == 1) { P.op(OP_DRIFT_EXP); P.arg(dl); }
             else if (model == 2) { P.op(OP_DRIFT_EXACT); P.arg(dl); }
             else { err = "drift model not supported"; ok = false; }
-            P.op(OP_GLOBAL_APER);
+            global_aper = true;
             break; }
         case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND:
         case XTM_SEXTUPOLE: case XTM_OCTUPOLE:
@@ -651,11 +685,39 @@ struct Builder {
             P.op(OP_LIMIT_ELLIPSE);
             for (int i = 0; i < 3; i++) P.arg(p[i]);
             break;
+        case XTM_DIPOLEEDGE: {
+            // dipoleedge.h:17-50: linear model r21, r43 scaled by
+            // (1 + delta_taper), negated when backtracking
+            const int64_t model = (int64_t)p[XTM_DE_MODEL];
+            if (model == 0) {
+                double r21 = p[XTM_DE_R21] * (1 + p[XTM_DE_DELTA_TAPER]);
+                double r43 = p[XTM_DE_R43] * (1 + p[XTM_DE_DELTA_TAPER]);
+                if (bt) { r21 = -r21; r43 = -r43; }
+                P.op(OP_EDGE_LINEAR); P.arg(r21); P.arg(r43);
+            } else if (model != -1) {
+                err = "dipole edge model " + std::to_string(model) + " is not supported";
+                ok = false;
+            }
+            break; }
+        case XTM_XYSHIFTED: ok = xy_shifted(p); break;
         default:
             err = "unknown element type " + std::to_string(type);
             ok = false;
         }
-        if (!ok) return -1;
+        return ok;
+    }
+
+    // element -> ops; returns start offset or -1.  The global aperture
+    // check of statically thick classes (tracker.py:681-689) follows the
+    // element and its frame transformations.
+    int64_t element(int type, const double *p) {
+        const int64_t start = (int64_t)prog.size();
+        Prog P(prog);
+        pending_desc = false;
+        pending_kicks.clear();
+        global_aper = false;
+        if (!body(type, p)) return -1;
+        if (global_aper) P.op(OP_GLOBAL_APER);
         P.op(OP_END);
         if (pending_desc) {
             size_t desc = kick_desc(pending_m, pending_c, pending_core_length, pending_factor);
@@ -664,6 +726,7 @@ struct Builder {
         return start;
     }
 
+    bool global_aper = false;
     bool pending_desc = false;
     std::vector<std::pair<size_t, double>> pending_kicks;
     MagnetArgs pending_m{};
@@ -680,8 +743,11 @@ size_t raw_size(int type, const double *p) {
         int64_t order = (int64_t)p[XTM_M_ORDER], nrel = (int64_t)p[XTM_M_NREL];
         return XTM_M_ARRAYS + 2 * (order + 1) + 2 * nrel; }
     case XTM_CAVITY: return XTM_C_SIZE;
+    case XTM_DIPOLEEDGE: return XTM_DE_SIZE;
     case XTM_LIMITRECT: return XTM_LR_SIZE;
     case XTM_LIMITELLIPSE: return XTM_LE_SIZE;
+    case XTM_XYSHIFTED:
+        return XTM_XS_INNER + raw_size((int)p[XTM_XS_INNER_TYPE], p + XTM_XS_INNER);
     default: return 0;
     }
 }

@@ -142,6 +142,7 @@ struct Gen {
                 line("xt::op_limit_ellipse(p" + cs(a, 3) + ", c.ignore_local());");
                 break;
             case OP_GLOBAL_APER: line("xt::op_global_aper(p, c.ignore_global());"); break;
+            case OP_XY_SHIFT: line("xt::op_xy_shift(p" + cs(a, 2) + ");"); break;
             default: return false;
             }
             pc += 1 + n;

@@ -7,7 +7,7 @@
 
 // Largest number of distinct element records for which line code is
 // generated; lines with more run on the program interpreter.
-constexpr int64_t XT_CODEGEN_MAX_RECORDS = 512;
+constexpr int64_t XT_CODEGEN_MAX_RECORDS = 2048;
 
 // Source of a translation unit that tracks the line: one function per
 // distinct element record with the record's constants as literals, an

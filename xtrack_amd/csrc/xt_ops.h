@@ -776,6 +776,14 @@ XT_HD void op_global_aper(PState<R> &p, const bool ignore_global) {
     if (p.state > 0 && !within) p.state = -1;
 }
 
+/* track_xyshift.h (add_to_x(-dx), add_to_y(-dy) with the negation done
+   when the program was built) */
+template <class R>
+XT_HD void op_xy_shift(PState<R> &p, const double ax, const double ay) {
+    p.x = p.x + ax;
+    p.y = p.y + ay;
+}
+
 struct TrackCtx {
     uint64_t flags;
     double line_length, q0, t_sim;
@@ -852,6 +860,10 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
         case OP_GLOBAL_APER:
             op_global_aper(p, c.ignore_global());
             pc += 1;
+            break;
+        case OP_XY_SHIFT:
+            op_xy_shift(p, a[0], a[1]);
+            pc += 3;
             break;
         default:
             p.state = -999;   /* corrupt program: never produced by the builder */

@@ -37,12 +37,13 @@ enum xt_op {
     OP_LIMIT_RECT = 12,    /* min_x max_x min_y max_y                          */
     OP_LIMIT_ELLIPSE = 13, /* a_squ b_squ a_b_squ                              */
     OP_GLOBAL_APER = 14,   /*                                                  */
-    OP_NUM = 15
+    OP_XY_SHIFT = 15,      /* ax ay   (x = x + ax, y = y + ay)                 */
+    OP_NUM = 16
 };
 
 /* number of constants following each op word */
 #define XT_NARGS_KICK 6
-#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0}
+#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2}
 
 /* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
  * OP_KICK through an absolute offset into prog:

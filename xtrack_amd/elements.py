@@ -8,6 +8,7 @@ parameter derivation of the reference classes:
   Sextupole     sextupole.py, Octupole octupole.py (same pattern as Quadrupole)
   Cavity        cavity.py:21-171
   LimitRect     limit_rect.py, LimitEllipse limit_ellipse.py:12-80
+  DipoleEdge    dipole_edge.py:12-150
   Marker        marker.py
 
 Each element serialises itself into the raw element table read by the HIP
@@ -25,6 +26,11 @@ from .constants import UNLIMITED
 # element type ids (include/xtrack_amd.h)
 T_MARKER, T_DRIFT, T_MULTIPOLE, T_QUADRUPOLE, T_BEND = 0, 1, 2, 3, 4
 T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, T_LIMITRECT, T_LIMITELLIPSE = 5, 6, 7, 8, 9
+T_DIPOLEEDGE, T_XYSHIFTED = 10, 11
+
+# classes whose transverse shifts are tracked (straight frame; Drift and
+# Marker do not allow shifts in the reference, bends need the curved frame)
+_SHIFTABLE = (T_MULTIPOLE, T_QUADRUPOLE, T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, 8, 9, 10)
 
 DEFAULT_MULTIPOLE_ORDER = 5          # _common.py:18
 
@@ -72,11 +78,19 @@ class BeamElement:
             setattr(self, nn, float(kwargs.pop(nn, 0.0)))
 
     def _check_misalignment(self):
-        for nn in _MISALIGN[:-1]:
+        """True when the element is transversely shifted.  Only shift_x and
+        shift_y are tracked (anchor 0, straight frame); any other non-zero
+        misalignment field raises."""
+        for nn in _MISALIGN[2:]:
             if getattr(self, nn) != 0.0:
                 raise NotImplementedError(
                     f'{type(self).__name__}: non-zero {nn} is not supported by the '
                     'HIP tracker')
+        shifted = self.shift_x != 0.0 or self.shift_y != 0.0
+        if shifted and self._type not in _SHIFTABLE:
+            raise NotImplementedError(
+                f'{type(self).__name__}: transverse shifts are not supported for this class')
+        return shifted
 
     def copy(self):
         out = type(self).__new__(type(self))
@@ -498,9 +512,84 @@ class LimitEllipse(BeamElement):
         return T_LIMITELLIPSE, [self.a_squ, self.b_squ, self.a_b_squ]
 
 
+class DipoleEdge(BeamElement):
+    """Thin dipole edge (dipole_edge.py).  The linear model's kick
+    coefficients r21, r43 are derived from k, e1, e1_fd, hgap and fint
+    exactly as the reference derives them (numpy, dipole_edge.py:123-131)."""
+    _type = T_DIPOLEEDGE
+    _MODELS = {'linear': 0, 'full': 1, 'suppressed': -1}
+    _SIDES = {'entry': 0, 'exit': 1}
+
+    def __init__(self, k=None, e1=None, e1_fd=None, hgap=None, fint=None, model=None,
+                 side=None, **kwargs):
+        self._init_misalignment(kwargs)
+        if 'h' in kwargs:                       # backward-compatible name
+            if k is not None:
+                raise ValueError('k and h given')
+            k = kwargs.pop('h')
+        self.delta_taper = float(kwargs.pop('delta_taper', 0.0))
+        for nn in ('r21', 'r43'):               # derived; recomputed below
+            kwargs.pop(nn, None)
+        if kwargs:
+            raise NameError(f'Invalid argument(s): {set(kwargs)}')
+        self._k = float(k) if k is not None else 0.0
+        self._e1 = float(e1) if e1 is not None else 0.0
+        self._e1_fd = float(e1_fd) if e1_fd is not None else 0.0
+        self._hgap = float(hgap) if hgap is not None else 0.0
+        self._fint = float(fint) if fint is not None else 0.0
+        self._model = self._MODELS['linear'] if model is None else _lookup(self._MODELS, model, 'model')
+        self._side = 0 if side is None else _lookup(self._SIDES, side, 'side')
+        self._update_r21_r43()
+
+    def _update_r21_r43(self):
+        k, e1, hgap, fint = (np.float64(v) for v in (self._k, self._e1, self._hgap, self._fint))
+        corr = np.float64(2.0) * k * hgap * fint
+        r21 = k * np.tan(e1)
+        e1_v = e1 + np.float64(self._e1_fd)
+        temp = corr / np.cos(e1_v) * (np.float64(1) + np.sin(e1_v) * np.sin(e1_v))
+        r43 = -k * np.tan(e1_v - temp)
+        self.r21, self.r43 = float(r21), float(r43)
+
+    def _prop(name):
+        def get(self):
+            return getattr(self, '_' + name)
+
+        def set_(self, value):
+            setattr(self, '_' + name, float(value))
+            self._update_r21_r43()
+        return property(get, set_)
+
+    k = _prop('k')
+    e1 = _prop('e1')
+    e1_fd = _prop('e1_fd')
+    hgap = _prop('hgap')
+    fint = _prop('fint')
+    del _prop
+
+    @property
+    def model(self):
+        return _name_of(self._MODELS, self._model)
+
+    @property
+    def side(self):
+        return _name_of(self._SIDES, self._side)
+
+    def to_dict(self):
+        return {'__class__': 'DipoleEdge', 'k': self._k, 'e1': self._e1, 'e1_fd': self._e1_fd,
+                'hgap': self._hgap, 'fint': self._fint, 'model': self.model,
+                'side': self.side, 'delta_taper': self.delta_taper}
+
+    def raw(self):
+        return T_DIPOLEEDGE, [self.r21, self.r43, float(self._model), self.delta_taper]
+
+
 ELEMENT_CLASSES = {c.__name__: c for c in (Marker, Drift, Multipole, Quadrupole, Bend,
                                            Sextupole, Octupole, Cavity, LimitRect,
-                                           LimitEllipse)}
+                                           LimitEllipse, DipoleEdge)}
+
+# keys of serialised elements that carry no tracking parameter (naming and
+# aperture bookkeeping of the reference's environment)
+_METADATA_KEYS = ('prototype', 'name_associated_aperture', 'parent_name', 'extra')
 
 
 def build_raw_table(elements):
@@ -514,9 +603,13 @@ def build_raw_table(elements):
     for ii, ee in enumerate(elements):
         key = id(ee)
         if key not in cache:
-            ee._check_misalignment()
+            shifted = ee._check_misalignment()
             tt, pp = ee.raw()
-            cache[key] = (tt, np.asarray(pp, dtype=np.float64))
+            pp = np.asarray(pp, dtype=np.float64)
+            if shifted:
+                pp = np.concatenate([[ee.shift_x, ee.shift_y, float(tt)], pp])
+                tt = T_XYSHIFTED
+            cache[key] = (tt, pp)
         tt, pp = cache[key]
         types[ii] = tt
         offsets[ii] = pos

@@ -103,12 +103,46 @@ class Line:
 
     @classmethod
     def from_dict(cls, dct):
+        """Line from the reference's serialised form (line.py:384,
+        Line.to_dict / json): `elements` (name -> element dict with
+        `__class__`) and `element_names`; a `particle_ref` dict, when
+        present, becomes `line.particle_ref`.  Element classes outside this
+        tracker raise NotImplementedError."""
+        from .elements import _METADATA_KEYS
         names = dct.get('element_names') or list(dct['elements'].keys())
         elems = []
+        built = {}
         for nn in names:
+            if nn in built:                    # a name may repeat in the sequence
+                elems.append(built[nn])
+                continue
             ed = dict(dct['elements'][nn])
             cls_name = ed.pop('__class__')
             if cls_name not in ELEMENT_CLASSES:
-                raise NotImplementedError(f'element class {cls_name} is not supported')
-            elems.append(ELEMENT_CLASSES[cls_name](**ed))
-        return cls(elements=elems, element_names=names)
+                raise NotImplementedError(f'element class {cls_name} ({nn}) is not supported')
+            for kk in _METADATA_KEYS:
+                ed.pop(kk, None)
+            built[nn] = ELEMENT_CLASSES[cls_name](**ed)
+            elems.append(built[nn])
+        line = cls(elements=elems, element_names=names)
+        if dct.get('particle_ref'):
+            line.particle_ref = Particles.from_dict(dct['particle_ref'])
+        return line
+
+    @classmethod
+    def from_json(cls, path):
+        """Load a line written by the reference's Line.to_json (plain or
+        gzip-compressed); a file holding {'line': ..., 'particle': ...} is
+        accepted too (the particle is then available as `line.json_particle`)."""
+        import gzip
+        import json
+        opener = gzip.open if str(path).endswith('.gz') else open
+        with opener(path, 'rt') as fh:
+            dct = json.load(fh)
+        particle = None
+        if 'line' in dct and 'elements' not in dct:
+            particle = dct.get('particle')
+            dct = dct['line']
+        line = cls.from_dict(dct)
+        line.json_particle = Particles.from_dict(particle) if particle else None
+        return line
