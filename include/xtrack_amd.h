@@ -46,7 +46,7 @@ extern "C" {
 #define XTM_LIMITRECT 8
 #define XTM_LIMITELLIPSE 9
 #define XTM_DIPOLEEDGE 10
-#define XTM_XYSHIFTED 11
+#define XTM_TRANSFORMED 11
 #define XTM_NUM_TYPES 12
 
 /* ---- raw parameter layout ---------------------------------------------------
@@ -133,17 +133,29 @@ extern "C" {
 #define XTM_DE_DELTA_TAPER 3
 #define XTM_DE_SIZE 4
 
-/* XYSHIFTED: an element displaced transversely (shift_x, shift_y; every
- * other misalignment field zero), wrapped as in
- * track_local_particle_with_transformations.h with the straight-frame
- * transforms of track_misalignments.h:38-110.  Record: shift_x, shift_y,
- * inner type, then the inner element's record.  Inner types: Multipole,
- * Quadrupole, Sextupole, Octupole, Cavity, LimitRect, LimitEllipse,
- * DipoleEdge. */
-#define XTM_XS_SHIFT_X 0
-#define XTM_XS_SHIFT_Y 1
-#define XTM_XS_INNER_TYPE 2
-#define XTM_XS_INNER 3
+/* TRANSFORMED: an element with non-zero misalignment fields, wrapped as in
+ * track_local_particle_with_transformations.h:88-168 with the frame
+ * transforms of track_misalignments.h (straight :38-112, curved :115-386).
+ * Record: the element's shift_x, shift_y, shift_s, rot_x_rad, rot_y_rad,
+ * rot_s_rad_no_frame, rot_shift_anchor, rot_s_rad; its length (0 for
+ * classes that are not thick), angle and h; 1 for classes that curve the
+ * reference frame (Bend), else 0; the inner type; then the inner element's
+ * record.  Inner types: Multipole, Quadrupole, Bend, Sextupole, Octupole,
+ * Cavity, LimitRect, LimitEllipse, DipoleEdge. */
+#define XTM_TR_SHIFT_X 0
+#define XTM_TR_SHIFT_Y 1
+#define XTM_TR_SHIFT_S 2
+#define XTM_TR_ROT_X 3
+#define XTM_TR_ROT_Y 4
+#define XTM_TR_ROT_S_NO_FRAME 5
+#define XTM_TR_ANCHOR 6
+#define XTM_TR_ROT_S 7
+#define XTM_TR_LENGTH 8
+#define XTM_TR_ANGLE 9
+#define XTM_TR_H 10
+#define XTM_TR_CURVED 11
+#define XTM_TR_INNER_TYPE 12
+#define XTM_TR_INNER 13
 
 /* ---- track flags (xtrack/track_flags.py:5-12) ------------------------------ */
 #define XTM_FLAG_BACKTRACK 0

@@ -1015,47 +1015,385 @@ static int is_static_thick(int type) {
 
 static int track_element_body(LP *lp, int type, const double *p);
 
-/* track_local_particle_with_transformations.h:80-132 with the straight
-   transforms of track_misalignments.h:38-110, for an element displaced by
-   shift_x, shift_y only (anchor, rotations and shift_s zero) */
-static int xyshifted_track(LP *part0, const double *p) {
-    double const dx = p[XTM_XS_SHIFT_X], dy = p[XTM_XS_SHIFT_Y];
-    int const inner = (int)p[XTM_XS_INNER_TYPE];
-    const double *q = p + XTM_XS_INNER;
-    double length = 0.0;
-    if (inner == XTM_MULTIPOLE) length = q[XTM_M_ISTHICK] > 0 ? q[XTM_M_LENGTH] : 0.0;
-    else if (inner == XTM_QUADRUPOLE || inner == XTM_SEXTUPOLE || inner == XTM_OCTUPOLE)
-        length = q[XTM_M_LENGTH];
-    else if (inner == XTM_CAVITY) length = q[XTM_C_LENGTH];
-    else if (!(inner == XTM_LIMITRECT || inner == XTM_LIMITELLIPSE || inner == XTM_DIPOLEEDGE))
-        return XTM_ERR_UNSUPPORTED;
-    double const anchor = 0.0, theta = 0.0, phi = 0.0;
-    double const mis_x_in = dx - anchor * M_COS(phi) * M_SIN(theta);
-    double const mis_y_in = dy - anchor * M_SIN(phi);
-    double const neg_part_length = anchor - length;
-    double const mis_x_out = neg_part_length * M_COS(phi) * M_SIN(theta) - dx;
-    double const mis_y_out = neg_part_length * M_SIN(phi) - dy;
+/* --------------------------------------------------------------------------
+ * Frame transformations of misaligned elements (track_misalignments.h) */
+
+/* track_xyshift.h:9-15 */
+static void xyshift_single(LP *part, double dx, double dy) {
+    ADD(x, -dx);
+    ADD(y, -dy);
+}
+
+/* track_srotation.h:11-40 (spin not tracked) */
+static void srotation_single(LP *part, double sin_z, double cos_z) {
+    double const x = GET(x);
+    double const y = GET(y);
+    double const px = GET(px);
+    double const py = GET(py);
+    double const x_hat = cos_z * x + sin_z * y;
+    double const y_hat = -sin_z * x + cos_z * y;
+    double const px_hat = cos_z * px + sin_z * py;
+    double const py_hat = -sin_z * px + cos_z * py;
+    SET(x, x_hat);
+    SET(y, y_hat);
+    SET(px, px_hat);
+    SET(py, py_hat);
+}
+
+/* local_particle_custom_api.h:51-55 */
+static double get_pzeta(LP *part) { return GET(ptau) / GET(beta0); }
+
+/* track_xrotation.h:11-45 (spin not tracked) */
+static void xrotation_single(LP *part, double sin_angle, double cos_angle, double tan_angle) {
+    double const beta0 = GET(beta0);
+    double const x = GET(x);
+    double const y = GET(y);
+    double const px = GET(px);
+    double const py = GET(py);
+    double const t = GET(zeta) / beta0;
+    double const pt = get_pzeta(part) * beta0;
+    double pz = sqrt(1.0 + 2.0 * pt / beta0 + pt * pt - px * px - py * py);
+    double ptt = 1.0 - tan_angle * py / pz;
+    double y_hat = y / (cos_angle * ptt);
+    double py_hat = cos_angle * py + sin_angle * pz;
+    double x_hat = x + tan_angle * y * px / (pz * ptt);
+    double t_hat = t - tan_angle * y * (1.0 / beta0 + pt) / (pz * ptt);
+    SET(x, x_hat);
+    SET(py, py_hat);
+    SET(y, y_hat);
+    SET(zeta, t_hat * beta0);
+}
+
+/* track_yrotation.h:11-45 (spin not tracked) */
+static void yrotation_single(LP *part, double sin_angle, double cos_angle, double tan_angle) {
+    double const beta0 = GET(beta0);
+    double const x = GET(x);
+    double const y = GET(y);
+    double const px = GET(px);
+    double const py = GET(py);
+    double const t = GET(zeta) / beta0;
+    double const pt = get_pzeta(part) * beta0;
+    double pz = sqrt(1.0 + 2.0 * pt / beta0 + pt * pt - px * px - py * py);
+    double ptt = 1.0 + tan_angle * px / pz;
+    double x_hat = x / (cos_angle * ptt);
+    double px_hat = cos_angle * px - sin_angle * pz;
+    double y_hat = y - tan_angle * x * py / (pz * ptt);
+    double t_hat = t + tan_angle * x * (1.0 / beta0 + pt) / (pz * ptt);
+    SET(x, x_hat);
+    SET(px, px_hat);
+    SET(y, y_hat);
+    SET(zeta, t_hat * beta0);
+}
+
+/* The particles a transformation applies to.  The serial CPU and GPU
+ * contexts run every block of an element over the particles active when the
+ * element starts (track.h:20-31, 58-62; the active range shrinks only
+ * between elements, tracker.py:702): particles lost inside the element are
+ * transformed back too. */
+typedef struct {
+    LP *part0;
+    const int64_t *idx;
+    int64_t n;
+} Blk;
+
+#define FOR_BLK(b, part)                                                      \
+    for (int64_t _k = 0; _k < (b)->n; _k++) {                                \
+        LP _lp = *(b)->part0; LP *part = &_lp; part->ipart = (b)->idx[_k];
+#define END_FOR_BLK }
+
+/* track_misalignments.h:16-30 */
+static void XY_SHIFT(const Blk *b, double dx, double dy) {
+    FOR_BLK(b, part) xyshift_single(part, dx, dy); END_FOR_BLK
+}
+static void S_SHIFT(const Blk *b, double ds) {
+    if (ds != 0.0) {
+        FOR_BLK(b, part) {
+            drift_exact(part, ds);
+            ADD(zeta, -ds);
+            ADD(s, -ds);
+        } END_FOR_BLK
+    }
+}
+static void Y_ROTATE(const Blk *b, double theta) {
+    if (theta != 0.0) {
+        FOR_BLK(b, part) yrotation_single(part, M_SIN(theta), M_COS(theta), M_TAN(theta));
+        END_FOR_BLK
+    }
+}
+static void X_ROTATE(const Blk *b, double phi) {
+    if (phi != 0.0) {
+        FOR_BLK(b, part) xrotation_single(part, -M_SIN(phi), M_COS(phi), -M_TAN(phi));
+        END_FOR_BLK
+    }
+}
+static void S_ROTATE(const Blk *b, double psi) {
+    if (psi != 0.0) {
+        FOR_BLK(b, part) srotation_single(part, M_SIN(psi), M_COS(psi)); END_FOR_BLK
+    }
+}
+
+/* track_misalignments.h:38-74 */
+static void misalignment_entry_straight(const Blk *b, double dx, double dy, double ds,
+                                        double theta, double phi, double psi_no_frame,
+                                        double anchor, double length, double psi_with_frame,
+                                        int backtrack) {
+    (void)length;
+    const double mis_x = dx - anchor * M_COS(phi) * M_SIN(theta);
+    const double mis_y = dy - anchor * M_SIN(phi);
+    const double mis_s = ds - anchor * (M_COS(phi) * M_COS(theta) - 1);
+    if (!backtrack) {
+        XY_SHIFT(b, mis_x, mis_y);
+        S_SHIFT(b, mis_s);
+        Y_ROTATE(b, theta);
+        X_ROTATE(b, phi);
+        S_ROTATE(b, psi_no_frame);
+        S_ROTATE(b, psi_with_frame);
+    } else {
+        S_ROTATE(b, -psi_with_frame);
+        S_ROTATE(b, -psi_no_frame);
+        X_ROTATE(b, -phi);
+        Y_ROTATE(b, -theta);
+        S_SHIFT(b, -mis_s);
+        XY_SHIFT(b, -mis_x, -mis_y);
+    }
+}
+
+/* track_misalignments.h:77-112 */
+static void misalignment_exit_straight(const Blk *b, double dx, double dy, double ds,
+                                       double theta, double phi, double psi_no_frame,
+                                       double anchor, double length, double psi_with_frame,
+                                       int backtrack) {
+    const double neg_part_length = anchor - length;
+    const double mis_x = neg_part_length * M_COS(phi) * M_SIN(theta) - dx;
+    const double mis_y = neg_part_length * M_SIN(phi) - dy;
+    const double mis_s = neg_part_length * (M_COS(phi) * M_COS(theta) - 1) - ds;
+    if (!backtrack) {
+        S_ROTATE(b, -psi_with_frame);
+        S_ROTATE(b, -psi_no_frame);
+        X_ROTATE(b, -phi);
+        Y_ROTATE(b, -theta);
+        S_SHIFT(b, mis_s);
+        XY_SHIFT(b, mis_x, mis_y);
+    } else {
+        XY_SHIFT(b, -mis_x, -mis_y);
+        S_SHIFT(b, -mis_s);
+        Y_ROTATE(b, theta);
+        X_ROTATE(b, phi);
+        S_ROTATE(b, psi_no_frame);
+        S_ROTATE(b, psi_with_frame);
+    }
+}
+
+/* track_misalignments.h:389-398 */
+static void matrix_multiply_4x4(const double a[4][4], const double b[4][4], double result[4][4]) {
+    for (int i = 0; i < 4; i++) {
+        result[i][0] = a[i][0] * b[0][0] + a[i][1] * b[1][0] + a[i][2] * b[2][0] + a[i][3] * b[3][0];
+        result[i][1] = a[i][0] * b[0][1] + a[i][1] * b[1][1] + a[i][2] * b[2][1] + a[i][3] * b[3][1];
+        result[i][2] = a[i][0] * b[0][2] + a[i][1] * b[1][2] + a[i][2] * b[2][2] + a[i][3] * b[3][2];
+        result[i][3] = a[i][0] * b[0][3] + a[i][1] * b[1][3] + a[i][2] * b[2][3] + a[i][3] * b[3][3];
+    }
+}
+
+/* track_misalignments.h:401-429 */
+static void matrix_rigid_affine_inverse(const double m[4][4], double inv_m[4][4]) {
+    const double m00 = m[0][0], m01 = m[0][1], m02 = m[0][2], m03 = m[0][3];
+    const double m10 = m[1][0], m11 = m[1][1], m12 = m[1][2], m13 = m[1][3];
+    const double m20 = m[2][0], m21 = m[2][1], m22 = m[2][2], m23 = m[2][3];
+    inv_m[0][0] = m00; inv_m[0][1] = m10; inv_m[0][2] = m20;
+    inv_m[1][0] = m01; inv_m[1][1] = m11; inv_m[1][2] = m21;
+    inv_m[2][0] = m02; inv_m[2][1] = m12; inv_m[2][2] = m22;
+    inv_m[0][3] = -m00 * m03 - m10 * m13 - m20 * m23;
+    inv_m[1][3] = -m01 * m03 - m11 * m13 - m21 * m23;
+    inv_m[2][3] = -m02 * m03 - m12 * m13 - m22 * m23;
+    inv_m[3][0] = 0; inv_m[3][1] = 0; inv_m[3][2] = 0; inv_m[3][3] = 1;
+}
+
+static void misalignment_matrix(double dx, double dy, double ds, double theta, double phi,
+                                double psi_no_frame, double m[4][4]) {   /* :139-174 */
+    const double s_phi = M_SIN(phi), c_phi = M_COS(phi);
+    const double s_theta = M_SIN(theta), c_theta = M_COS(theta);
+    const double s_psi = M_SIN(psi_no_frame), c_psi = M_COS(psi_no_frame);
+    m[0][0] = -s_phi * s_psi * s_theta + c_psi * c_theta;
+    m[0][1] = -c_psi * s_phi * s_theta - c_theta * s_psi;
+    m[0][2] = c_phi * s_theta;
+    m[0][3] = dx;
+    m[1][0] = c_phi * s_psi;
+    m[1][1] = c_phi * c_psi;
+    m[1][2] = s_phi;
+    m[1][3] = dy;
+    m[2][0] = -c_theta * s_phi * s_psi - c_psi * s_theta;
+    m[2][1] = -c_psi * c_theta * s_phi + s_psi * s_theta;
+    m[2][2] = c_phi * c_theta;
+    m[2][3] = ds;
+    m[3][0] = 0; m[3][1] = 0; m[3][2] = 0; m[3][3] = 1;
+}
+
+/* track_misalignments.h:115-241 */
+static void misalignment_entry_curved(const Blk *b, double dx, double dy, double ds,
+                                      double theta, double phi, double psi_no_frame,
+                                      double anchor, double length, double angle, double h,
+                                      double psi_with_frame, int backtrack) {
+    if (angle == 0.0 && (length != 0.0 || h == 0.0)) {
+        misalignment_entry_straight(b, dx, dy, ds, theta, phi, psi_no_frame, anchor, length,
+                                    psi_with_frame, backtrack);
+        return;
+    }
+    double misalignment[4][4];
+    misalignment_matrix(dx, dy, ds, theta, phi, psi_no_frame, misalignment);
+    if (length != 0.0) h = angle / length;
+    const double part_angle = anchor * h;
+    const double delta_x_first_part = (M_COS(part_angle) - 1) * M_COS(psi_with_frame) / h;
+    const double delta_y_first_part = (M_COS(part_angle) - 1) * M_SIN(psi_with_frame) / h;
+    const double delta_s_first_part = M_SIN(part_angle) / h;
+    const double first[4][4] = {
+        {(M_COS(part_angle) - 1) * POW2(M_COS(psi_with_frame)) + 1,
+         (M_COS(part_angle) - 1) * M_COS(psi_with_frame) * M_SIN(psi_with_frame),
+         -M_COS(psi_with_frame) * M_SIN(part_angle), delta_x_first_part},
+        {(M_COS(part_angle) - 1) * M_COS(psi_with_frame) * M_SIN(psi_with_frame),
+         (M_COS(part_angle) - 1) * POW2(M_SIN(psi_with_frame)) + 1,
+         -M_SIN(part_angle) * M_SIN(psi_with_frame), delta_y_first_part},
+        {M_COS(psi_with_frame) * M_SIN(part_angle), M_SIN(part_angle) * M_SIN(psi_with_frame),
+         M_COS(part_angle), delta_s_first_part},
+        {0, 0, 0, 1}};
+    double inv_first[4][4], misaligned_entry[4][4], temp[4][4];
+    matrix_rigid_affine_inverse(first, inv_first);
+    matrix_multiply_4x4(first, misalignment, temp);
+    matrix_multiply_4x4(temp, inv_first, misaligned_entry);
+    const double mis_x = misaligned_entry[0][3];
+    const double mis_y = misaligned_entry[1][3];
+    const double mis_s = misaligned_entry[2][3];
+    const double rot_theta = atan2(misaligned_entry[0][2], misaligned_entry[2][2]);
+    const double rot_phi = atan2(misaligned_entry[1][2],
+                                 sqrt(misaligned_entry[1][0] * misaligned_entry[1][0] +
+                                      misaligned_entry[1][1] * misaligned_entry[1][1]));
+    const double rot_psi = atan2(misaligned_entry[1][0], misaligned_entry[1][1]);
+    if (!backtrack) {
+        XY_SHIFT(b, mis_x, mis_y);
+        S_SHIFT(b, mis_s);
+        Y_ROTATE(b, rot_theta);
+        X_ROTATE(b, rot_phi);
+        S_ROTATE(b, rot_psi);
+        S_ROTATE(b, psi_with_frame);
+    } else {
+        S_ROTATE(b, -psi_with_frame);
+        S_ROTATE(b, -rot_psi);
+        X_ROTATE(b, -rot_phi);
+        Y_ROTATE(b, -rot_theta);
+        S_SHIFT(b, -mis_s);
+        XY_SHIFT(b, -mis_x, -mis_y);
+    }
+}
+
+/* track_misalignments.h:244-386 */
+static void misalignment_exit_curved(const Blk *b, double dx, double dy, double ds,
+                                     double theta, double phi, double psi_no_frame,
+                                     double anchor, double length, double angle, double h,
+                                     double psi_with_frame, int backtrack) {
+    if (angle == 0.0 && (length != 0.0 || h == 0.0)) {
+        misalignment_exit_straight(b, dx, dy, ds, theta, phi, psi_no_frame, anchor, length,
+                                   psi_with_frame, backtrack);
+        return;
+    }
+    double misalignment[4][4], inv_misalignment[4][4];
+    misalignment_matrix(dx, dy, ds, theta, phi, psi_no_frame, misalignment);
+    matrix_rigid_affine_inverse(misalignment, inv_misalignment);
+    if (length != 0.0) h = angle / length;
+    const double part_angle = angle - h * anchor;
+    const double s_part_angle = M_SIN(part_angle), c_part_angle = M_COS(part_angle);
+    const double s_tilt = M_SIN(psi_with_frame), c_tilt = M_COS(psi_with_frame);
+    const double delta_x_second_part = (c_part_angle - 1) * c_tilt / h;
+    const double delta_y_second_part = (c_part_angle - 1) * s_tilt / h;
+    const double delta_s_second_part = s_part_angle / h;
+    const double second[4][4] = {
+        {(c_part_angle - 1) * POW2(c_tilt) + 1, (c_part_angle - 1) * c_tilt * s_tilt,
+         c_tilt * -s_part_angle, delta_x_second_part},
+        {(c_part_angle - 1) * c_tilt * s_tilt, (c_part_angle - 1) * POW2(s_tilt) + 1,
+         -s_part_angle * s_tilt, delta_y_second_part},
+        {c_tilt * s_part_angle, s_part_angle * s_tilt, c_part_angle, delta_s_second_part},
+        {0, 0, 0, 1}};
+    double inv_second[4][4], realign[4][4], temp[4][4];
+    matrix_rigid_affine_inverse(second, inv_second);
+    matrix_multiply_4x4(inv_second, inv_misalignment, temp);
+    matrix_multiply_4x4(temp, second, realign);
+    double mis_x = realign[0][3];
+    double mis_y = realign[1][3];
+    double mis_s = realign[2][3];
+    double rot_theta = atan2(realign[0][2], realign[2][2]);
+    double rot_phi = atan2(realign[1][2],
+                           sqrt(realign[1][0] * realign[1][0] + realign[1][1] * realign[1][1]));
+    double rot_psi = atan2(realign[1][0], realign[1][1]);
+    if (!backtrack) {
+        S_ROTATE(b, -psi_with_frame);
+        XY_SHIFT(b, mis_x, mis_y);
+        S_SHIFT(b, mis_s);
+        Y_ROTATE(b, rot_theta);
+        X_ROTATE(b, rot_phi);
+        S_ROTATE(b, rot_psi);
+    } else {
+        S_ROTATE(b, -rot_psi);
+        X_ROTATE(b, -rot_phi);
+        Y_ROTATE(b, -rot_theta);
+        S_SHIFT(b, -mis_s);
+        XY_SHIFT(b, -mis_x, -mis_y);
+        S_ROTATE(b, psi_with_frame);
+    }
+}
+
+/* *_track_local_particle_with_nonzero_transformations
+ * (track_local_particle_with_transformations.h:88-168): the record holds
+ * the element's misalignment fields, its length, angle and h, whether the
+ * class curves the reference frame (CURVED), and the element itself. */
+static int transformed_track(LP *part0, const double *p) {
+    double const shift_x = p[XTM_TR_SHIFT_X];
+    double const shift_y = p[XTM_TR_SHIFT_Y];
+    double const shift_s = p[XTM_TR_SHIFT_S];
+    double const rot_x_rad = p[XTM_TR_ROT_X];
+    double const rot_y_rad = p[XTM_TR_ROT_Y];
+    double const rot_s_rad_no_frame = p[XTM_TR_ROT_S_NO_FRAME];
+    double const anchor = p[XTM_TR_ANCHOR];
+    double const rot_s_rad = p[XTM_TR_ROT_S];
+    double const length = p[XTM_TR_LENGTH];
+    double const angle = p[XTM_TR_ANGLE];
+    double const h = p[XTM_TR_H];
+    int const curved = p[XTM_TR_CURVED] != 0.0;
+    int const inner = (int)p[XTM_TR_INNER_TYPE];
+    const double *q = p + XTM_TR_INNER;
+    if (!(inner >= XTM_MULTIPOLE && inner <= XTM_DIPOLEEDGE)) return XTM_ERR_UNSUPPORTED;
     int const backtrack = check_flag(part0, XTM_FLAG_BACKTRACK);
-    /* entry (exit with backtrack): XY_SHIFT adds -dx', -dy' */
-    double const pre_x = backtrack ? -(-mis_x_out) : -mis_x_in;
-    double const pre_y = backtrack ? -(-mis_y_out) : -mis_y_in;
-    double const post_x = backtrack ? -(-mis_x_in) : -mis_x_out;
-    double const post_y = backtrack ? -(-mis_y_in) : -mis_y_out;
-    /* The serial CPU and GPU contexts run every block of an element over
-     * the particles active when the element starts (track.h:20-31, 58-62;
-     * the active range shrinks only between elements, tracker.py:702):
-     * particles lost inside the element are shifted back too. */
+
     int64_t const n = part0->endpart - part0->ipart;
     int64_t *active = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     if (!active) return XTM_ERR_NOMEM;
     int64_t na = 0;
-    FOR_ACTIVE(part0, part) { active[na++] = part->ipart; ADD(x, pre_x); ADD(y, pre_y); }
-    END_FOR_ACTIVE
+    FOR_ACTIVE(part0, part) { active[na++] = part->ipart; } END_FOR_ACTIVE
+    Blk b = {part0, active, na};
+
+#define ENTRY(bt)                                                                              \
+    do {                                                                                       \
+        if (curved)                                                                            \
+            misalignment_entry_curved(&b, shift_x, shift_y, shift_s, rot_y_rad, rot_x_rad,     \
+                                      rot_s_rad_no_frame, anchor, length, angle, h, rot_s_rad, \
+                                      bt);                                                     \
+        else                                                                                   \
+            misalignment_entry_straight(&b, shift_x, shift_y, shift_s, rot_y_rad, rot_x_rad,   \
+                                        rot_s_rad_no_frame, anchor, length, rot_s_rad, bt);    \
+    } while (0)
+#define EXIT(bt)                                                                               \
+    do {                                                                                       \
+        if (curved)                                                                            \
+            misalignment_exit_curved(&b, shift_x, shift_y, shift_s, rot_y_rad, rot_x_rad,      \
+                                     rot_s_rad_no_frame, anchor, length, angle, h, rot_s_rad,  \
+                                     bt);                                                      \
+        else                                                                                   \
+            misalignment_exit_straight(&b, shift_x, shift_y, shift_s, rot_y_rad, rot_x_rad,    \
+                                       rot_s_rad_no_frame, anchor, length, rot_s_rad, bt);     \
+    } while (0)
+
+    if (!backtrack) ENTRY(0); else EXIT(1);
     int st = track_element_body(part0, inner, q);
-    for (int64_t k = 0; k < na; k++) {
-        LP lp = *part0; LP *part = &lp; part->ipart = active[k];
-        ADD(x, post_x); ADD(y, post_y);
-    }
+    if (!backtrack) EXIT(0); else ENTRY(1);
+#undef ENTRY
+#undef EXIT
     free(active);
     return st;
 }
@@ -1071,7 +1409,7 @@ static int track_element_body(LP *lp, int type, const double *p) {
     case XTM_LIMITRECT: limitrect_track(lp, p); break;
     case XTM_LIMITELLIPSE: limitellipse_track(lp, p); break;
     case XTM_DIPOLEEDGE: st = dipoleedge_track(lp, p); break;
-    case XTM_XYSHIFTED: st = xyshifted_track(lp, p); break;
+    case XTM_TRANSFORMED: st = transformed_track(lp, p); break;
     default: return XTM_ERR_ARG;
     }
     return st;
@@ -1081,7 +1419,7 @@ static int track_element(LP *lp, int type, const double *p) {
     int st = track_element_body(lp, type, p);
     if (st) return st;
     /* the class's static thickness decides the global check (tracker.py:681) */
-    int const cls = type == XTM_XYSHIFTED ? (int)p[XTM_XS_INNER_TYPE] : type;
+    int const cls = type == XTM_TRANSFORMED ? (int)p[XTM_TR_INNER_TYPE] : type;
     if (is_static_thick(cls)) global_aperture_check(lp);
     return XTM_OK;
 }

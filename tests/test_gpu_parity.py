@@ -234,24 +234,14 @@ def test_sps_lattice_bit_exact():
     assert not bad, bad
 
 
-def test_shifted_elements_bit_exact():
-    print('[shifted elements]', flush=True)
-
-    def mk():
-        return xt.Line(elements=[
-            xt.Drift(length=0.5),
-            xt.Quadrupole(length=0.8, k1=0.4, shift_x=1e-3, shift_y=-2e-3),
-            xt.Sextupole(length=0.3, k2=2.0, shift_y=5e-4),
-            xt.Octupole(length=0.2, k3=50.0, shift_x=-3e-4),
-            xt.Multipole(knl=[1e-4, 0.02], length=0.5, isthick=True, hxl=1e-4, shift_x=1e-3),
-            xt.Cavity(voltage=5e6, frequency=400e6, lag=175, length=1.0, shift_x=1e-3),
-            xt.DipoleEdge(k=0.01, e1=0.05, shift_y=1e-3),
-            xt.LimitRect(min_x=-0.01, max_x=0.01, min_y=-0.01, max_y=0.01, shift_x=3e-3),
-            xt.LimitEllipse(a=0.01, b=0.008, shift_y=-2e-3),
-            xt.Drift(length=0.5)])
+def test_misaligned_elements_bit_exact():
+    """Shifts, rolls, pitch and yaw with anchors on every class that allows
+    them, straight and curved frames, forward and backtracking."""
+    from test_misalignments import _misaligned_line
+    print('[misaligned elements]', flush=True)
     p = random_particles(3000, seed=12, scale=5e-3)
     for bt in (False, True):
-        lg, lc, pg, pc = _pair(mk, p)
+        lg, lc, pg, pc = _pair(lambda: xt.Line(elements=_misaligned_line()), p)
         lg.track(pg, num_turns=4, backtrack=bt)
         lc.track(pc, num_turns=4, backtrack=bt)
         bad = compare_exact(pg, pc)

@@ -72,16 +72,23 @@ def test_builder_rejects_unsupported_models(el):
 
 
 def test_misalignment_table_build():
-    # transverse shifts wrap the element (track_misalignments.h:38-110)
-    q = xt.Quadrupole(length=1.0, k1=0.1, shift_x=1e-3)
+    # misaligned elements are wrapped (track_local_particle_with_transformations.h)
+    q = xt.Quadrupole(length=1.0, k1=0.1, shift_x=1e-3, rot_s_rad=0.2)
     types, offsets, params = build_raw_table([q])
-    assert list(types) == [xt.elements.T_XYSHIFTED]
-    assert list(params[:3]) == [1e-3, 0.0, float(xt.elements.T_QUADRUPOLE)]
-    # other misalignments, and shifts of classes without straight-frame
-    # transformations, are rejected
-    for ee in (xt.Quadrupole(length=1.0, k1=0.1, rot_s_rad=1e-3),
-               xt.Quadrupole(length=1.0, k1=0.1, shift_s=1e-3),
-               xt.Bend(length=1.0, angle=1e-3, shift_y=1e-3)):
+    assert list(types) == [xt.elements.T_TRANSFORMED]
+    assert list(params[:13]) == [1e-3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.2, 1.0, 0.0, 0.0, 0.0,
+                                 float(xt.elements.T_QUADRUPOLE)]
+    b = xt.Bend(length=2.0, angle=0.1, rot_y_rad=1e-3, rot_shift_anchor=1.0)
+    _, _, params = build_raw_table([b])
+    assert list(params[3:13]) == [0.0, 1e-3, 0.0, 1.0, 0.0, 2.0, 0.1, 0.05, 1.0,
+                                  float(xt.elements.T_BEND)]
+    # thin multipole: no length; the anchor alone does not switch them on
+    m = xt.Multipole(knl=[0, 1e-3], length=0.5, shift_y=1e-3)
+    assert build_raw_table([m])[2][8] == 0.0
+    assert build_raw_table([xt.Multipole(knl=[0, 1.0], rot_shift_anchor=0.3)])[0][0] == \
+        xt.elements.T_MULTIPOLE
+    # Drift and Marker do not allow shifts or rotations
+    for ee in (xt.Drift(length=1.0, shift_x=1e-3), xt.Marker(rot_s_rad=0.1)):
         with pytest.raises(NotImplementedError):
             build_raw_table([ee])
 

@@ -48,9 +48,9 @@ def test_sps_json_load():
     assert line.json_particle.p0c[0] == 25983064590.009007
     # the one shifted aperture becomes a wrapped record
     types, _, _ = build_raw_table(line.elements)
-    assert Counter(types.tolist())[xt.elements.T_XYSHIFTED] == 1
+    assert Counter(types.tolist())[xt.elements.T_TRANSFORMED] == 1
     i = line.element_names.index('tpst.21760_aper')
-    assert types[i] == xt.elements.T_XYSHIFTED
+    assert types[i] == xt.elements.T_TRANSFORMED
 
 
 def test_dipole_edge_linear_coefficients():

@@ -619,40 +619,257 @@ struct Builder {
         return true;
     }
 
-    // track_misalignments.h:38-110 for a straight frame with only the
-    // transverse shifts non-zero (anchor, rotations, shift_s zero): the
-    // entry adds -mis_x, the exit -mis_x' (reversed when backtracking)
-    bool xy_shifted(const double *p) {
-        const double dx = p[XTM_XS_SHIFT_X], dy = p[XTM_XS_SHIFT_Y];
-        const int inner = (int)p[XTM_XS_INNER_TYPE];
-        const double *q = p + XTM_XS_INNER;
-        double length = 0.0;   // IS_THICK / IS_THICK_DYNAMIC elements
+    // ---- frame transformations (track_misalignments.h) ----------------------
+    // XY_SHIFT(dx, dy): add -dx, -dy; skipped only where both additions
+    // are of -0.0, which leave every value unchanged
+    void xy_shift(double dx, double dy) {
+        const double ax = -dx, ay = -dy;
+        if (std::signbit(ax) && ax == 0.0 && std::signbit(ay) && ay == 0.0) return;
+        Prog P(prog);
+        P.op(OP_XY_SHIFT); P.arg(ax); P.arg(ay);
+    }
+    void s_shift(double ds) {            // IF_NONZERO(DS, ...)
+        if (ds == 0.0) return;
+        Prog P(prog);
+        P.op(OP_S_SHIFT); P.arg(ds);
+    }
+    void y_rotate(double theta) {        // YRotation(sin, cos, tan)
+        if (theta == 0.0) return;
+        Prog P(prog);
+        P.op(OP_Y_ROT); P.arg(xt_sin(theta)); P.arg(xt_cos(theta)); P.arg(xt_tan(theta));
+    }
+    void x_rotate(double phi) {          // XRotation(-sin, cos, -tan)
+        if (phi == 0.0) return;
+        Prog P(prog);
+        P.op(OP_X_ROT); P.arg(-xt_sin(phi)); P.arg(xt_cos(phi)); P.arg(-xt_tan(phi));
+    }
+    void s_rotate(double psi) {          // SRotation(sin, cos)
+        if (psi == 0.0) return;
+        Prog P(prog);
+        P.op(OP_S_ROT); P.arg(xt_sin(psi)); P.arg(xt_cos(psi));
+    }
+
+    struct Misalign {
+        double dx, dy, ds, theta, phi, psi_no_frame, anchor, length, angle, h, psi_with_frame;
+    };
+
+    // track_misalignments.h:38-74
+    void entry_straight(const Misalign &g, bool backtrack) {
+        const double mis_x = g.dx - g.anchor * xt_cos(g.phi) * xt_sin(g.theta);
+        const double mis_y = g.dy - g.anchor * xt_sin(g.phi);
+        const double mis_s = g.ds - g.anchor * (xt_cos(g.phi) * xt_cos(g.theta) - 1);
+        if (!backtrack) {
+            xy_shift(mis_x, mis_y);
+            s_shift(mis_s);
+            y_rotate(g.theta);
+            x_rotate(g.phi);
+            s_rotate(g.psi_no_frame);
+            s_rotate(g.psi_with_frame);
+        } else {
+            s_rotate(-g.psi_with_frame);
+            s_rotate(-g.psi_no_frame);
+            x_rotate(-g.phi);
+            y_rotate(-g.theta);
+            s_shift(-mis_s);
+            xy_shift(-mis_x, -mis_y);
+        }
+    }
+
+    // track_misalignments.h:77-112
+    void exit_straight(const Misalign &g, bool backtrack) {
+        const double neg_part_length = g.anchor - g.length;
+        const double mis_x = neg_part_length * xt_cos(g.phi) * xt_sin(g.theta) - g.dx;
+        const double mis_y = neg_part_length * xt_sin(g.phi) - g.dy;
+        const double mis_s = neg_part_length * (xt_cos(g.phi) * xt_cos(g.theta) - 1) - g.ds;
+        if (!backtrack) {
+            s_rotate(-g.psi_with_frame);
+            s_rotate(-g.psi_no_frame);
+            x_rotate(-g.phi);
+            y_rotate(-g.theta);
+            s_shift(mis_s);
+            xy_shift(mis_x, mis_y);
+        } else {
+            xy_shift(-mis_x, -mis_y);
+            s_shift(-mis_s);
+            y_rotate(g.theta);
+            x_rotate(g.phi);
+            s_rotate(g.psi_no_frame);
+            s_rotate(g.psi_with_frame);
+        }
+    }
+
+    typedef double M4[4][4];
+    // track_misalignments.h:389-398
+    static void mat_mul(const M4 a, const M4 b, M4 r) {
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                r[i][j] = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j] +
+                          a[i][3] * b[3][j];
+    }
+    // track_misalignments.h:401-429
+    static void mat_rigid_inverse(const M4 m, M4 inv) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) inv[i][j] = m[j][i];
+        for (int i = 0; i < 3; i++)
+            inv[i][3] = -m[0][i] * m[0][3] - m[1][i] * m[1][3] - m[2][i] * m[2][3];
+        inv[3][0] = 0; inv[3][1] = 0; inv[3][2] = 0; inv[3][3] = 1;
+    }
+    // the misalignment matrix (track_misalignments.h:155-174)
+    static void misalignment_matrix(const Misalign &g, M4 m) {
+        const double s_phi = xt_sin(g.phi), c_phi = xt_cos(g.phi);
+        const double s_theta = xt_sin(g.theta), c_theta = xt_cos(g.theta);
+        const double s_psi = xt_sin(g.psi_no_frame), c_psi = xt_cos(g.psi_no_frame);
+        const M4 t = {
+            {-s_phi * s_psi * s_theta + c_psi * c_theta, -c_psi * s_phi * s_theta - c_theta * s_psi,
+             c_phi * s_theta, g.dx},
+            {c_phi * s_psi, c_phi * c_psi, s_phi, g.dy},
+            {-c_theta * s_phi * s_psi - c_psi * s_theta, -c_psi * c_theta * s_phi + s_psi * s_theta,
+             c_phi * c_theta, g.ds},
+            {0, 0, 0, 1}};
+        std::memcpy(m, t, sizeof(M4));
+    }
+    // basic transformations of a rigid frame change (:215-222, :354-361)
+    static void decompose(const M4 e, double *mx, double *my, double *ms, double *theta,
+                          double *phi, double *psi) {
+        *mx = e[0][3];
+        *my = e[1][3];
+        *ms = e[2][3];
+        *theta = std::atan2(e[0][2], e[2][2]);
+        *phi = std::atan2(e[1][2], std::sqrt(e[1][0] * e[1][0] + e[1][1] * e[1][1]));
+        *psi = std::atan2(e[1][0], e[1][1]);
+    }
+
+    static bool straight_frame(const Misalign &g) {     // :131-137, :245-251
+        return g.angle == 0.0 && (g.length != 0.0 || g.h == 0.0);
+    }
+
+    // track_misalignments.h:115-241
+    void entry_curved(const Misalign &g, bool backtrack) {
+        if (straight_frame(g)) { entry_straight(g, backtrack); return; }
+        M4 mis;
+        misalignment_matrix(g, mis);
+        double h = g.h;
+        if (g.length != 0.0) h = g.angle / g.length;
+        const double part_angle = g.anchor * h;
+        const double pwf = g.psi_with_frame;
+        const double delta_x_first_part = (xt_cos(part_angle) - 1) * xt_cos(pwf) / h;
+        const double delta_y_first_part = (xt_cos(part_angle) - 1) * xt_sin(pwf) / h;
+        const double delta_s_first_part = xt_sin(part_angle) / h;
+        const M4 first = {
+            {(xt_cos(part_angle) - 1) * (xt_cos(pwf) * xt_cos(pwf)) + 1,
+             (xt_cos(part_angle) - 1) * xt_cos(pwf) * xt_sin(pwf), -xt_cos(pwf) * xt_sin(part_angle),
+             delta_x_first_part},
+            {(xt_cos(part_angle) - 1) * xt_cos(pwf) * xt_sin(pwf),
+             (xt_cos(part_angle) - 1) * (xt_sin(pwf) * xt_sin(pwf)) + 1,
+             -xt_sin(part_angle) * xt_sin(pwf), delta_y_first_part},
+            {xt_cos(pwf) * xt_sin(part_angle), xt_sin(part_angle) * xt_sin(pwf), xt_cos(part_angle),
+             delta_s_first_part},
+            {0, 0, 0, 1}};
+        M4 inv_first, temp, e;
+        mat_rigid_inverse(first, inv_first);
+        mat_mul(first, mis, temp);
+        mat_mul(temp, inv_first, e);
+        double mis_x, mis_y, mis_s, rot_theta, rot_phi, rot_psi;
+        decompose(e, &mis_x, &mis_y, &mis_s, &rot_theta, &rot_phi, &rot_psi);
+        if (!backtrack) {
+            xy_shift(mis_x, mis_y);
+            s_shift(mis_s);
+            y_rotate(rot_theta);
+            x_rotate(rot_phi);
+            s_rotate(rot_psi);
+            s_rotate(pwf);
+        } else {
+            s_rotate(-pwf);
+            s_rotate(-rot_psi);
+            x_rotate(-rot_phi);
+            y_rotate(-rot_theta);
+            s_shift(-mis_s);
+            xy_shift(-mis_x, -mis_y);
+        }
+    }
+
+    // track_misalignments.h:244-386
+    void exit_curved(const Misalign &g, bool backtrack) {
+        if (straight_frame(g)) { exit_straight(g, backtrack); return; }
+        M4 mis, inv_mis;
+        misalignment_matrix(g, mis);
+        mat_rigid_inverse(mis, inv_mis);
+        double h = g.h;
+        if (g.length != 0.0) h = g.angle / g.length;
+        const double part_angle = g.angle - h * g.anchor;
+        const double s_part_angle = xt_sin(part_angle), c_part_angle = xt_cos(part_angle);
+        const double s_tilt = xt_sin(g.psi_with_frame), c_tilt = xt_cos(g.psi_with_frame);
+        const double delta_x_second_part = (c_part_angle - 1) * c_tilt / h;
+        const double delta_y_second_part = (c_part_angle - 1) * s_tilt / h;
+        const double delta_s_second_part = s_part_angle / h;
+        const M4 second = {
+            {(c_part_angle - 1) * (c_tilt * c_tilt) + 1, (c_part_angle - 1) * c_tilt * s_tilt,
+             c_tilt * -s_part_angle, delta_x_second_part},
+            {(c_part_angle - 1) * c_tilt * s_tilt, (c_part_angle - 1) * (s_tilt * s_tilt) + 1,
+             -s_part_angle * s_tilt, delta_y_second_part},
+            {c_tilt * s_part_angle, s_part_angle * s_tilt, c_part_angle, delta_s_second_part},
+            {0, 0, 0, 1}};
+        M4 inv_second, temp, realign;
+        mat_rigid_inverse(second, inv_second);
+        mat_mul(inv_second, inv_mis, temp);
+        mat_mul(temp, second, realign);
+        double mis_x, mis_y, mis_s, rot_theta, rot_phi, rot_psi;
+        decompose(realign, &mis_x, &mis_y, &mis_s, &rot_theta, &rot_phi, &rot_psi);
+        if (!backtrack) {
+            s_rotate(-g.psi_with_frame);
+            xy_shift(mis_x, mis_y);
+            s_shift(mis_s);
+            y_rotate(rot_theta);
+            x_rotate(rot_phi);
+            s_rotate(rot_psi);
+        } else {
+            s_rotate(-rot_psi);
+            x_rotate(-rot_phi);
+            y_rotate(-rot_theta);
+            s_shift(-mis_s);
+            xy_shift(-mis_x, -mis_y);
+            s_rotate(g.psi_with_frame);
+        }
+    }
+
+    // track_local_particle_with_transformations.h:88-168: the entry
+    // transformation, the element in its own frame, the exit
+    // transformation; when backtracking the exit's inverse comes first
+    bool transformed(const double *p) {
+        const int inner = (int)p[XTM_TR_INNER_TYPE];
         switch (inner) {
-        case XTM_MULTIPOLE: length = q[XTM_M_ISTHICK] > 0 ? q[XTM_M_LENGTH] : 0.0; break;
-        case XTM_QUADRUPOLE: case XTM_SEXTUPOLE: case XTM_OCTUPOLE:
-            length = q[XTM_M_LENGTH]; break;
-        case XTM_CAVITY: length = q[XTM_C_LENGTH]; break;
-        case XTM_LIMITRECT: case XTM_LIMITELLIPSE: case XTM_DIPOLEEDGE: break;
+        case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND: case XTM_SEXTUPOLE:
+        case XTM_OCTUPOLE: case XTM_CAVITY: case XTM_LIMITRECT: case XTM_LIMITELLIPSE:
+        case XTM_DIPOLEEDGE: break;
         default:
-            err = "transverse shifts are not supported for element type " + std::to_string(inner);
+            err = "frame transformations are not supported for element type " +
+                  std::to_string(inner);
             return false;
         }
-        const double anchor = 0.0, theta = 0.0, phi = 0.0, ds = 0.0;
-        const double mis_x_in = dx - anchor * xt_cos(phi) * xt_sin(theta);
-        const double mis_y_in = dy - anchor * xt_sin(phi);
-        const double neg_part_length = anchor - length;
-        const double mis_x_out = neg_part_length * xt_cos(phi) * xt_sin(theta) - dx;
-        const double mis_y_out = neg_part_length * xt_sin(phi) - dy;
-        const double mis_s_in = ds - anchor * (xt_cos(phi) * xt_cos(theta) - 1);
-        const double mis_s_out = neg_part_length * (xt_cos(phi) * xt_cos(theta) - 1) - ds;
-        if (mis_s_in != 0.0 || mis_s_out != 0.0) { err = "longitudinal shift"; return false; }
-        Prog P(prog);
-        // XYShift_single_particle(dx', dy') adds -dx', -dy'
-        if (!bt) { P.op(OP_XY_SHIFT); P.arg(-mis_x_in); P.arg(-mis_y_in); }
-        else { P.op(OP_XY_SHIFT); P.arg(-(-mis_x_out)); P.arg(-(-mis_y_out)); }
-        if (!body(inner, q)) return false;
-        if (!bt) { P.op(OP_XY_SHIFT); P.arg(-mis_x_out); P.arg(-mis_y_out); }
-        else { P.op(OP_XY_SHIFT); P.arg(-(-mis_x_in)); P.arg(-(-mis_y_in)); }
+        Misalign g;
+        g.dx = p[XTM_TR_SHIFT_X];
+        g.dy = p[XTM_TR_SHIFT_Y];
+        g.ds = p[XTM_TR_SHIFT_S];
+        g.theta = p[XTM_TR_ROT_Y];
+        g.phi = p[XTM_TR_ROT_X];
+        g.psi_no_frame = p[XTM_TR_ROT_S_NO_FRAME];
+        g.anchor = p[XTM_TR_ANCHOR];
+        g.psi_with_frame = p[XTM_TR_ROT_S];
+        g.length = p[XTM_TR_LENGTH];
+        g.angle = p[XTM_TR_ANGLE];
+        g.h = p[XTM_TR_H];
+        const bool curved = p[XTM_TR_CURVED] != 0.0;
+        if (!bt) {
+            if (curved) entry_curved(g, false); else entry_straight(g, false);
+        } else {
+            if (curved) exit_curved(g, true); else exit_straight(g, true);
+        }
+        if (!body(inner, p + XTM_TR_INNER)) return false;
+        if (!bt) {
+            if (curved) exit_curved(g, false); else exit_straight(g, false);
+        } else {
+            if (curved) entry_curved(g, true); else entry_straight(g, true);
+        }
         return true;
     }
 
@@ -699,7 +916,7 @@ struct Builder {
                 ok = false;
             }
             break; }
-        case XTM_XYSHIFTED: ok = xy_shifted(p); break;
+        case XTM_TRANSFORMED: ok = transformed(p); break;
         default:
             err = "unknown element type " + std::to_string(type);
             ok = false;
@@ -746,8 +963,10 @@ size_t raw_size(int type, const double *p) {
     case XTM_DIPOLEEDGE: return XTM_DE_SIZE;
     case XTM_LIMITRECT: return XTM_LR_SIZE;
     case XTM_LIMITELLIPSE: return XTM_LE_SIZE;
-    case XTM_XYSHIFTED:
-        return XTM_XS_INNER + raw_size((int)p[XTM_XS_INNER_TYPE], p + XTM_XS_INNER);
+    case XTM_TRANSFORMED: {
+        const int inner = (int)p[XTM_TR_INNER_TYPE];
+        if (inner == XTM_TRANSFORMED) return 0;
+        return XTM_TR_INNER + raw_size(inner, p + XTM_TR_INNER); }
     default: return 0;
     }
 }
@@ -777,6 +996,15 @@ int xt_build_program(const int32_t *types, const int64_t *offsets, const double 
         if (type >= XTM_MULTIPOLE && type <= XTM_OCTUPOLE) {
             if (p[XTM_M_ORDER] < 0 || p[XTM_M_NREL] < 1) {
                 err = "element " + std::to_string(i) + ": invalid multipole arrays";
+                return XTM_ERR_ARG;
+            }
+        }
+        if (type == XTM_TRANSFORMED) {
+            const double it = p[XTM_TR_INNER_TYPE];
+            const double *q = p + XTM_TR_INNER;
+            if (!(it >= XTM_MULTIPOLE && it <= XTM_DIPOLEEDGE) ||
+                (it <= XTM_OCTUPOLE && (q[XTM_M_ORDER] < 0 || q[XTM_M_NREL] < 1))) {
+                err = "element " + std::to_string(i) + ": invalid transformed record";
                 return XTM_ERR_ARG;
             }
         }

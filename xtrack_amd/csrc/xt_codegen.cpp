@@ -143,6 +143,10 @@ struct Gen {
                 break;
             case OP_GLOBAL_APER: line("xt::op_global_aper(p, c.ignore_global());"); break;
             case OP_XY_SHIFT: line("xt::op_xy_shift(p" + cs(a, 2) + ");"); break;
+            case OP_S_SHIFT: line("xt::op_s_shift(p" + cs(a, 1) + ");"); break;
+            case OP_X_ROT: line("xt::op_x_rot(p, m" + cs(a, 3) + ");"); break;
+            case OP_Y_ROT: line("xt::op_y_rot(p, m" + cs(a, 3) + ");"); break;
+            case OP_S_ROT: line("xt::op_s_rot(p" + cs(a, 2) + ");"); break;
             default: return false;
             }
             pc += 1 + n;

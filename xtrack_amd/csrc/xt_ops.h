@@ -784,6 +784,88 @@ XT_HD void op_xy_shift(PState<R> &p, const double ax, const double ay) {
     p.y = p.y + ay;
 }
 
+/* S_SHIFT of track_misalignments.h:26-30: Drift_single_particle_exact over
+   ds (track_drift.h:26-41), then zeta and s moved back by ds */
+template <class R>
+XT_HD void op_s_shift(PState<R> &p, const double ds) {
+    drift_exact(p, ds);
+    p.zeta = p.zeta + (-ds);
+    p.s = p.s + (-ds);
+}
+
+/* track_srotation.h:11-40 (spin is not tracked) */
+template <class R>
+XT_HD void op_s_rot(PState<R> &p, const double sin_z, const double cos_z) {
+    const R x = p.x, y = p.y, px = p.px, py = p.py;
+    p.x = cos_z * x + sin_z * y;
+    p.y = -sin_z * x + cos_z * y;
+    p.px = cos_z * px + sin_z * py;
+    p.py = -sin_z * px + cos_z * py;
+}
+
+template <class R>
+struct RotOut { R x, y, p, zeta; };
+
+/* track_xrotation.h:11-45 (spin is not tracked): returns x, y, py, zeta.
+   Out of line: frame rotations are rare and need ptau and beta0 from
+   memory. */
+template <class R>
+XT_HD_COLD RotOut<R> x_rotation(const R x, const R y, const R px, const R py, const R zeta,
+                                const R ptau, const R beta0, const double sin_angle,
+                                const double cos_angle, const double tan_angle) {
+    const R t = zeta / beta0;
+    const R pt = (ptau / beta0) * beta0;   /* LocalParticle_get_pzeta * beta0 */
+    const R pz = xsqrt(1.0 + 2.0 * pt / beta0 + pt * pt - px * px - py * py);
+    const R ptt = 1.0 - tan_angle * py / pz;
+    RotOut<R> o;
+    o.y = y / (cos_angle * ptt);
+    o.p = cos_angle * py + sin_angle * pz;
+    o.x = x + tan_angle * y * px / (pz * ptt);
+    const R t_hat = t - tan_angle * y * (1.0 / beta0 + pt) / (pz * ptt);
+    o.zeta = t_hat * beta0;
+    return o;
+}
+
+/* track_yrotation.h:11-45 (spin is not tracked): returns x, y, px, zeta */
+template <class R>
+XT_HD_COLD RotOut<R> y_rotation(const R x, const R y, const R px, const R py, const R zeta,
+                                const R ptau, const R beta0, const double sin_angle,
+                                const double cos_angle, const double tan_angle) {
+    const R t = zeta / beta0;
+    const R pt = (ptau / beta0) * beta0;
+    const R pz = xsqrt(1.0 + 2.0 * pt / beta0 + pt * pt - px * px - py * py);
+    const R ptt = 1.0 + tan_angle * px / pz;
+    RotOut<R> o;
+    o.x = x / (cos_angle * ptt);
+    o.p = cos_angle * px - sin_angle * pz;
+    o.y = y - tan_angle * x * py / (pz * ptt);
+    const R t_hat = t + tan_angle * x * (1.0 / beta0 + pt) / (pz * ptt);
+    o.zeta = t_hat * beta0;
+    return o;
+}
+
+template <class R>
+XT_HD void op_x_rot(PState<R> &p, const PRef &m, const double sin_angle, const double cos_angle,
+                    const double tan_angle) {
+    const RotOut<R> o = x_rotation(p.x, p.y, p.px, p.py, p.zeta, R(m.P->ptau[m.i]),
+                                   R(m.P->beta0[m.i]), sin_angle, cos_angle, tan_angle);
+    p.x = o.x;
+    p.py = o.p;
+    p.y = o.y;
+    p.zeta = o.zeta;
+}
+
+template <class R>
+XT_HD void op_y_rot(PState<R> &p, const PRef &m, const double sin_angle, const double cos_angle,
+                    const double tan_angle) {
+    const RotOut<R> o = y_rotation(p.x, p.y, p.px, p.py, p.zeta, R(m.P->ptau[m.i]),
+                                   R(m.P->beta0[m.i]), sin_angle, cos_angle, tan_angle);
+    p.x = o.x;
+    p.px = o.p;
+    p.y = o.y;
+    p.zeta = o.zeta;
+}
+
 struct TrackCtx {
     uint64_t flags;
     double line_length, q0, t_sim;
@@ -863,6 +945,22 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
             break;
         case OP_XY_SHIFT:
             op_xy_shift(p, a[0], a[1]);
+            pc += 3;
+            break;
+        case OP_S_SHIFT:
+            op_s_shift(p, a[0]);
+            pc += 2;
+            break;
+        case OP_X_ROT:
+            op_x_rot(p, m, a[0], a[1], a[2]);
+            pc += 4;
+            break;
+        case OP_Y_ROT:
+            op_y_rot(p, m, a[0], a[1], a[2]);
+            pc += 4;
+            break;
+        case OP_S_ROT:
+            op_s_rot(p, a[0], a[1]);
             pc += 3;
             break;
         default:

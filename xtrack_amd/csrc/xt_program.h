@@ -38,12 +38,16 @@ enum xt_op {
     OP_LIMIT_ELLIPSE = 13, /* a_squ b_squ a_b_squ                              */
     OP_GLOBAL_APER = 14,   /*                                                  */
     OP_XY_SHIFT = 15,      /* ax ay   (x = x + ax, y = y + ay)                 */
-    OP_NUM = 16
+    OP_S_SHIFT = 16,       /* ds      (exact drift ds, zeta and s back by ds)  */
+    OP_X_ROT = 17,         /* sin cos tan as XRotation_single_particle takes   */
+    OP_Y_ROT = 18,         /* sin cos tan as YRotation_single_particle takes   */
+    OP_S_ROT = 19,         /* sin cos                                          */
+    OP_NUM = 20
 };
 
 /* number of constants following each op word */
 #define XT_NARGS_KICK 6
-#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2}
+#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2, 1, 3, 3, 2}
 
 /* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
  * OP_KICK through an absolute offset into prog:

@@ -13,9 +13,10 @@ parameter derivation of the reference classes:
 
 Each element serialises itself into the raw element table read by the HIP
 library (layout in include/xtrack_amd.h).  Misalignment fields exist on
-every element as in the reference (base_element.py:274-282); non-zero
-values are rejected when a line is built because the shifted/rotated frame
-transformation is not on this tracker's path.
+every element as in the reference (base_element.py:274-282); an element
+with a non-zero one is wrapped in a TRANSFORMED record, tracked with the
+frame transformations of track_misalignments.h.  Drift and Marker do not
+allow them (allow_rot_and_shift = False in the reference).
 """
 import math
 
@@ -26,11 +27,7 @@ from .constants import UNLIMITED
 # element type ids (include/xtrack_amd.h)
 T_MARKER, T_DRIFT, T_MULTIPOLE, T_QUADRUPOLE, T_BEND = 0, 1, 2, 3, 4
 T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, T_LIMITRECT, T_LIMITELLIPSE = 5, 6, 7, 8, 9
-T_DIPOLEEDGE, T_XYSHIFTED = 10, 11
-
-# classes whose transverse shifts are tracked (straight frame; Drift and
-# Marker do not allow shifts in the reference, bends need the curved frame)
-_SHIFTABLE = (T_MULTIPOLE, T_QUADRUPOLE, T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, 8, 9, 10)
+T_DIPOLEEDGE, T_TRANSFORMED = 10, 11
 
 DEFAULT_MULTIPOLE_ORDER = 5          # _common.py:18
 
@@ -77,20 +74,35 @@ class BeamElement:
         for nn in _MISALIGN:
             setattr(self, nn, float(kwargs.pop(nn, 0.0)))
 
-    def _check_misalignment(self):
-        """True when the element is transversely shifted.  Only shift_x and
-        shift_y are tracked (anchor 0, straight frame); any other non-zero
-        misalignment field raises."""
-        for nn in _MISALIGN[2:]:
-            if getattr(self, nn) != 0.0:
-                raise NotImplementedError(
-                    f'{type(self).__name__}: non-zero {nn} is not supported by the '
-                    'HIP tracker')
-        shifted = self.shift_x != 0.0 or self.shift_y != 0.0
-        if shifted and self._type not in _SHIFTABLE:
+    # track_local_particle_with_transformations.h:170-204: the frame
+    # transformations run when any of these is non-zero (the anchor alone
+    # does not switch them on)
+    _ACTIVE_MISALIGN = ('shift_x', 'shift_y', 'shift_s', 'rot_x_rad', 'rot_y_rad', 'rot_s_rad',
+                        'rot_s_rad_no_frame')
+
+    def _transform_prefix(self):
+        """None for an aligned element, else the TRANSFORMED record prefix:
+        the misalignment fields, the element's length (IS_THICK /
+        IS_THICK_DYNAMIC classes), angle and h, and whether the class
+        curves the reference frame (base_element.py:83-125)."""
+        if not any(getattr(self, nn) != 0.0 for nn in self._ACTIVE_MISALIGN):
+            return None
+        if self._type in (T_DRIFT, T_MARKER):
             raise NotImplementedError(
-                f'{type(self).__name__}: transverse shifts are not supported for this class')
-        return shifted
+                f'{type(self).__name__} does not allow shifts or rotations')
+        if self._type == T_MULTIPOLE:
+            length = self.length if self.isthick else 0.0
+        elif self.isthick:
+            length = self.length
+        else:
+            length = 0.0
+        curved = self._type == T_BEND
+        angle = self.angle if curved else 0.0
+        h = self.h if curved else 0.0
+        return [self.shift_x, self.shift_y, self.shift_s, self.rot_x_rad, self.rot_y_rad,
+                self.rot_s_rad_no_frame, self.rot_shift_anchor, self.rot_s_rad,
+                float(length), float(angle), float(h), 1.0 if curved else 0.0,
+                float(self._type)]
 
     def copy(self):
         out = type(self).__new__(type(self))
@@ -603,12 +615,12 @@ def build_raw_table(elements):
     for ii, ee in enumerate(elements):
         key = id(ee)
         if key not in cache:
-            shifted = ee._check_misalignment()
+            prefix = ee._transform_prefix()
             tt, pp = ee.raw()
             pp = np.asarray(pp, dtype=np.float64)
-            if shifted:
-                pp = np.concatenate([[ee.shift_x, ee.shift_y, float(tt)], pp])
-                tt = T_XYSHIFTED
+            if prefix is not None:
+                pp = np.concatenate([prefix, pp])
+                tt = T_TRANSFORMED
             cache[key] = (tt, pp)
         tt, pp = cache[key]
         types[ii] = tt
