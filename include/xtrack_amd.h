@@ -47,7 +47,8 @@ extern "C" {
 #define XTM_LIMITELLIPSE 9
 #define XTM_DIPOLEEDGE 10
 #define XTM_TRANSFORMED 11
-#define XTM_NUM_TYPES 12
+#define XTM_RFMULTIPOLE 12
+#define XTM_NUM_TYPES 13
 
 /* ---- raw parameter layout ---------------------------------------------------
  * Element i occupies params[offsets[i] ...].  Integer fields are stored as
@@ -133,6 +134,16 @@ extern "C" {
 #define XTM_DE_DELTA_TAPER 3
 #define XTM_DE_SIZE 4
 
+/* RFMultipole (rf_multipole.py, rfmultipole.h -> track_rf.h:18-167 with
+ * model -1, one kick): voltage, frequency, lag (deg), phase (rad), order,
+ * then knl, ksl, pn (deg), ps (deg), phase_n, phase_s, each order+1 long. */
+#define XTM_RFM_VOLTAGE 0
+#define XTM_RFM_FREQUENCY 1
+#define XTM_RFM_LAG 2
+#define XTM_RFM_PHASE 3
+#define XTM_RFM_ORDER 4
+#define XTM_RFM_ARRAYS 5
+
 /* TRANSFORMED: an element with non-zero misalignment fields, wrapped as in
  * track_local_particle_with_transformations.h:88-168 with the frame
  * transforms of track_misalignments.h (straight :38-112, curved :115-386).
@@ -141,7 +152,7 @@ extern "C" {
  * classes that are not thick), angle and h; 1 for classes that curve the
  * reference frame (Bend), else 0; the inner type; then the inner element's
  * record.  Inner types: Multipole, Quadrupole, Bend, Sextupole, Octupole,
- * Cavity, LimitRect, LimitEllipse, DipoleEdge. */
+ * Cavity, LimitRect, LimitEllipse, DipoleEdge, RFMultipole. */
 #define XTM_TR_SHIFT_X 0
 #define XTM_TR_SHIFT_Y 1
 #define XTM_TR_SHIFT_S 2

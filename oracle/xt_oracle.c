@@ -777,10 +777,14 @@ static int magnet_track(LP *part0, int type, const double *p) {
 }
 
 /* --------------------------------------------------------------------------
- * Cavity: cavity.h:12-48 -> track_rf.h:317-453, kick :18-167 (order -1,
- * no transverse voltage), energy update with pz_only = 1. */
-static void rf_kick(LP *part, double voltage, double frequency, double harmonic, double lag,
-                    double phase, int64_t absolute_time, int kill_energy_kick) {
+ * RF kick (track_rf.h:18-167; no transverse voltage: the Cavity and the
+ * RFMultipole pass 0) with the energy update pz_only = 1 (custom_api.h). */
+static void rf_kick_single_particle(LP *part, double voltage, double frequency, double harmonic,
+                                    double lag, double phase, int64_t absolute_time,
+                                    int64_t order, double factor_knl_ksl, const double *knl,
+                                    const double *ksl, const double *pn, const double *ps,
+                                    const double *phase_n, const double *phase_s,
+                                    int kill_energy_kick) {
     double phase0 = 0;
     double const beta0 = GET(beta0);
     if (harmonic != 0) {
@@ -799,7 +803,85 @@ static void rf_kick(LP *part, double voltage, double frequency, double harmonic,
     double const energy_kick = q * voltage
         * M_SIN(phase0 + DEG2RAD * lag + phase - (2.0 * PI) / C_LIGHT * frequency * tau);
     double rfmultipole_energy_kick = 0;
+    if (order >= 0) {
+        double dpx = 0.0;
+        double dpy = 0.0;
+        double dptr = 0.0;
+        double zre = 1.0;
+        double zim = 0.0;
+        double factorial = 1.0;
+        double const x = GET(x);
+        double const y = GET(y);
+        double const p0c = GET(p0c);
+        for (int64_t kk = 0; kk <= order; kk++) {
+            if (kk > 0) factorial *= kk;
+            double const pn_kk = phase0 + DEG2RAD * pn[kk] + phase_n[kk]
+                                 - (2.0 * PI) / C_LIGHT * frequency * tau;
+            double const ps_kk = phase0 + DEG2RAD * ps[kk] + phase_s[kk]
+                                 - (2.0 * PI) / C_LIGHT * frequency * tau;
+            double bal_n_kk = factor_knl_ksl * knl[kk] / factorial;
+            double bal_s_kk = factor_knl_ksl * ksl[kk] / factorial;
+            double const cn = M_COS(pn_kk);
+            double const cs = M_COS(ps_kk);
+            double const sn = M_SIN(pn_kk);
+            double const ss = M_SIN(ps_kk);
+            dpx += cn * (bal_n_kk * zre) - cs * (bal_s_kk * zim);
+            dpy += cs * (bal_s_kk * zre) + cn * (bal_n_kk * zim);
+            double const zret = zre * x - zim * y;
+            zim = zim * x + zre * y;
+            zre = zret;
+            dptr += sn * (bal_n_kk * zre) - ss * (bal_s_kk * zim);
+        }
+        rfmultipole_energy_kick = -q * ((frequency * (2.0 * PI / C_LIGHT) * p0c) * dptr);
+        double const chi = GET(chi);
+        double const px_kick = -chi * dpx;
+        double const py_kick = chi * dpy;
+        ADD(px, px_kick);
+        ADD(py, py_kick);
+    }
     if (!kill_energy_kick) add_to_energy(part, energy_kick + rfmultipole_energy_kick, 1);
+}
+
+/* the Cavity's kick: order -1 (cavity.h:12-48) */
+static void rf_kick(LP *part, double voltage, double frequency, double harmonic, double lag,
+                    double phase, int64_t absolute_time, int kill_energy_kick) {
+    rf_kick_single_particle(part, voltage, frequency, harmonic, lag, phase, absolute_time, -1,
+                            1.0, NULL, NULL, NULL, NULL, NULL, NULL, kill_energy_kick);
+}
+
+/* RFMultipole: rfmultipole.h:11-47 -> track_rf.h:317-453 with weight 1,
+ * length 0, harmonic 0, absolute_time 0, model -1 (one uniform kick); the
+ * zero-length drifts around the kick do nothing. */
+static int rfmultipole_track(LP *part0, const double *p) {
+    double voltage = p[XTM_RFM_VOLTAGE];
+    double const frequency = p[XTM_RFM_FREQUENCY];
+    double const lag = p[XTM_RFM_LAG];
+    double const phase = p[XTM_RFM_PHASE];
+    int64_t const order = (int64_t)p[XTM_RFM_ORDER];
+    if (order < 0) return XTM_ERR_ARG;
+    const double *knl = p + XTM_RFM_ARRAYS;
+    const double *ksl = knl + (order + 1);
+    const double *pn = ksl + (order + 1);
+    const double *ps = pn + (order + 1);
+    const double *phase_n = ps + (order + 1);
+    const double *phase_s = phase_n + (order + 1);
+    double const factor_knl_ksl = 1.0;
+    double factor_knl_ksl_body;
+    int const kill_energy_kick = check_flag(part0, XTM_FLAG_KILL_CAVITY_KICK);
+    if (check_flag(part0, XTM_FLAG_BACKTRACK)) {
+        factor_knl_ksl_body = -factor_knl_ksl;
+        voltage = -voltage;
+    } else {
+        factor_knl_ksl_body = factor_knl_ksl;
+    }
+    double const weight = 1.;
+    double const kick_weight = 1. / 1;
+    FOR_ACTIVE(part0, part) {
+        rf_kick_single_particle(part, voltage * weight * (kick_weight), frequency, 0., lag, phase,
+                                0, order, factor_knl_ksl_body * weight * (kick_weight), knl, ksl,
+                                pn, ps, phase_n, phase_s, kill_energy_kick);
+    } END_FOR_ACTIVE
+    return XTM_OK;
 }
 
 static int cavity_track(LP *part0, const double *p) {
@@ -1358,7 +1440,8 @@ static int transformed_track(LP *part0, const double *p) {
     int const curved = p[XTM_TR_CURVED] != 0.0;
     int const inner = (int)p[XTM_TR_INNER_TYPE];
     const double *q = p + XTM_TR_INNER;
-    if (!(inner >= XTM_MULTIPOLE && inner <= XTM_DIPOLEEDGE)) return XTM_ERR_UNSUPPORTED;
+    if (!((inner >= XTM_MULTIPOLE && inner <= XTM_DIPOLEEDGE) || inner == XTM_RFMULTIPOLE))
+        return XTM_ERR_UNSUPPORTED;
     int const backtrack = check_flag(part0, XTM_FLAG_BACKTRACK);
 
     int64_t const n = part0->endpart - part0->ipart;
@@ -1410,6 +1493,7 @@ static int track_element_body(LP *lp, int type, const double *p) {
     case XTM_LIMITELLIPSE: limitellipse_track(lp, p); break;
     case XTM_DIPOLEEDGE: st = dipoleedge_track(lp, p); break;
     case XTM_TRANSFORMED: st = transformed_track(lp, p); break;
+    case XTM_RFMULTIPOLE: st = rfmultipole_track(lp, p); break;
     default: return XTM_ERR_ARG;
     }
     return st;

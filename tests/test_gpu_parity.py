@@ -247,3 +247,34 @@ def test_misaligned_elements_bit_exact():
         bad = compare_exact(pg, pc)
         assert (pc.state <= 0).sum() > 100
         assert not bad, (bt, bad)
+
+
+def test_rf_multipoles_bit_exact():
+    from test_rf_multipole import rf_line
+    print('[rf multipoles]', flush=True)
+    rng = np.random.default_rng(21)
+    n = 2000
+    p = xt.Particles(p0c=7e9, x=rng.normal(0, 4e-3, n), px=rng.normal(0, 1e-4, n),
+                     y=rng.normal(0, 4e-3, n), py=rng.normal(0, 1e-4, n),
+                     zeta=rng.normal(0, 0.2, n), delta=rng.normal(0, 1e-3, n),
+                     charge_ratio=rng.choice([1.0, 0.5], n), mass_ratio=rng.choice([1.0, 2.0], n))
+    for bt in (False, True):
+        lg, lc, pg, pc = _pair(lambda: xt.Line(elements=rf_line()), p)
+        lg.track(pg, num_turns=3, backtrack=bt)
+        lc.track(pc, num_turns=3, backtrack=bt)
+        bad = compare_exact(pg, pc)
+        assert not bad, (bt, bad)
+
+
+def test_hllhc_lattice_bit_exact():
+    """HL-LHC v1.4 of the reference's test data (beam-beam elements
+    replaced by markers): thin multipoles, crab cavities, cavities."""
+    from test_hllhc_lattice import hllhc_line, hllhc_particles
+    print('[hllhc lattice]', flush=True)
+    p = hllhc_particles(3000, seed=3)
+    lg, lc, pg, pc = _pair(hllhc_line, p)
+    lg.track(pg, num_turns=3)
+    lc.track(pc, num_turns=3)
+    bad = compare_exact(pg, pc)
+    assert (pc.sorted_by_id()['state'] <= 0).sum() > 100
+    assert not bad, bad

@@ -9,7 +9,8 @@ turn_by_turn_monitor)`` semantics, executed by a persistent HIP kernel.
 from .constants import PROTON_MASS_EV, ELECTRON_MASS_EV, C_LIGHT
 from .particles import Particles
 from .elements import (Marker, Drift, Multipole, Quadrupole, Bend, Sextupole,
-                       Octupole, Cavity, LimitRect, LimitEllipse, DipoleEdge)
+                       Octupole, Cavity, LimitRect, LimitEllipse, DipoleEdge,
+                       RFMultipole)
 from .monitor import ParticlesMonitor
 from .line import Line
 from .tracker import Tracker, TrackFlags, HIPBackend

@@ -26,6 +26,9 @@
 
 namespace {
 
+// DEG2RAD of headers/constants.h
+constexpr double kDeg2Rad = 0.0174532925199432957692369076848861271344287188854;
+
 double ibits(int64_t v) { double d; std::memcpy(&d, &v, 8); return d; }
 
 struct Prog {
@@ -619,6 +622,39 @@ struct Builder {
         return true;
     }
 
+    // rfmultipole.h:11-47 -> track_rf.h:317-453 with model -1: one kick of
+    // weight 1 between two zero-length drifts (skipped); backtracking
+    // negates the voltage and the multipole factor
+    bool rf_multipole(const double *p) {
+        const int64_t order = (int64_t)p[XTM_RFM_ORDER];
+        if (order < 0) { err = "RF multipole order < 0"; return false; }
+        const double factor = bt ? -1.0 : 1.0;
+        const double voltage = bt ? -p[XTM_RFM_VOLTAGE] : p[XTM_RFM_VOLTAGE];
+        const double *knl = p + XTM_RFM_ARRAYS, *ksl = knl + (order + 1);
+        const double *pn = ksl + (order + 1), *ps = pn + (order + 1);
+        const double *phase_n = ps + (order + 1), *phase_s = phase_n + (order + 1);
+        std::vector<double> d(RFD_TERMS + RFD_STRIDE * (order + 1));
+        d[RFD_ORDER] = ibits(order);
+        double factorial = 1.0;
+        for (int64_t kk = 0; kk <= order; kk++) {
+            if (kk > 0) factorial *= kk;
+            double *t = &d[RFD_TERMS + RFD_STRIDE * kk];
+            t[0] = kDeg2Rad * pn[kk];
+            t[1] = phase_n[kk];
+            t[2] = kDeg2Rad * ps[kk];
+            t[3] = phase_s[kk];
+            t[4] = factor * knl[kk] / factorial;
+            t[5] = factor * ksl[kk] / factorial;
+        }
+        Prog P(prog);
+        P.op(OP_RF_MULT);
+        P.arg(voltage * 1.0); P.arg(p[XTM_RFM_FREQUENCY]); P.arg(p[XTM_RFM_LAG]);
+        P.arg(p[XTM_RFM_PHASE]);
+        pending_rf.push_back({prog.size(), std::move(d)});
+        P.arg(0.0);   // descriptor offset, patched after OP_END
+        return true;
+    }
+
     // ---- frame transformations (track_misalignments.h) ----------------------
     // XY_SHIFT(dx, dy): add -dx, -dy; skipped only where both additions
     // are of -0.0, which leave every value unchanged
@@ -840,7 +876,7 @@ struct Builder {
         switch (inner) {
         case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND: case XTM_SEXTUPOLE:
         case XTM_OCTUPOLE: case XTM_CAVITY: case XTM_LIMITRECT: case XTM_LIMITELLIPSE:
-        case XTM_DIPOLEEDGE: break;
+        case XTM_DIPOLEEDGE: case XTM_RFMULTIPOLE: break;
         default:
             err = "frame transformations are not supported for element type " +
                   std::to_string(inner);
@@ -917,6 +953,7 @@ struct Builder {
             }
             break; }
         case XTM_TRANSFORMED: ok = transformed(p); break;
+        case XTM_RFMULTIPOLE: ok = rf_multipole(p); break;
         default:
             err = "unknown element type " + std::to_string(type);
             ok = false;
@@ -932,6 +969,7 @@ struct Builder {
         Prog P(prog);
         pending_desc = false;
         pending_kicks.clear();
+        pending_rf.clear();
         global_aper = false;
         if (!body(type, p)) return -1;
         if (global_aper) P.op(OP_GLOBAL_APER);
@@ -940,8 +978,14 @@ struct Builder {
             size_t desc = kick_desc(pending_m, pending_c, pending_core_length, pending_factor);
             for (auto &k : pending_kicks) prog[k.first] = ibits((int64_t)desc);
         }
+        for (auto &r : pending_rf) {
+            prog[r.first] = ibits((int64_t)prog.size());
+            prog.insert(prog.end(), r.second.begin(), r.second.end());
+        }
+        pending_rf.clear();
         return start;
     }
+    std::vector<std::pair<size_t, std::vector<double>>> pending_rf;
 
     bool global_aper = false;
     bool pending_desc = false;
@@ -963,6 +1007,7 @@ size_t raw_size(int type, const double *p) {
     case XTM_DIPOLEEDGE: return XTM_DE_SIZE;
     case XTM_LIMITRECT: return XTM_LR_SIZE;
     case XTM_LIMITELLIPSE: return XTM_LE_SIZE;
+    case XTM_RFMULTIPOLE: return XTM_RFM_ARRAYS + 6 * ((int64_t)p[XTM_RFM_ORDER] + 1);
     case XTM_TRANSFORMED: {
         const int inner = (int)p[XTM_TR_INNER_TYPE];
         if (inner == XTM_TRANSFORMED) return 0;
@@ -999,11 +1044,17 @@ int xt_build_program(const int32_t *types, const int64_t *offsets, const double 
                 return XTM_ERR_ARG;
             }
         }
+        if (type == XTM_RFMULTIPOLE && !(p[XTM_RFM_ORDER] >= 0)) {
+            err = "element " + std::to_string(i) + ": invalid RF multipole order";
+            return XTM_ERR_ARG;
+        }
         if (type == XTM_TRANSFORMED) {
             const double it = p[XTM_TR_INNER_TYPE];
             const double *q = p + XTM_TR_INNER;
-            if (!(it >= XTM_MULTIPOLE && it <= XTM_DIPOLEEDGE) ||
-                (it <= XTM_OCTUPOLE && (q[XTM_M_ORDER] < 0 || q[XTM_M_NREL] < 1))) {
+            if (!((it >= XTM_MULTIPOLE && it <= XTM_DIPOLEEDGE) || it == XTM_RFMULTIPOLE) ||
+                (it >= XTM_MULTIPOLE && it <= XTM_OCTUPOLE &&
+                 (q[XTM_M_ORDER] < 0 || q[XTM_M_NREL] < 1)) ||
+                (it == XTM_RFMULTIPOLE && !(q[XTM_RFM_ORDER] >= 0))) {
                 err = "element " + std::to_string(i) + ": invalid transformed record";
                 return XTM_ERR_ARG;
             }

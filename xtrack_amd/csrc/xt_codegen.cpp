@@ -147,6 +147,9 @@ struct Gen {
             case OP_X_ROT: line("xt::op_x_rot(p, m" + cs(a, 3) + ");"); break;
             case OP_Y_ROT: line("xt::op_y_rot(p, m" + cs(a, 3) + ");"); break;
             case OP_S_ROT: line("xt::op_s_rot(p" + cs(a, 2) + ");"); break;
+            case OP_RF_MULT:
+                line("xt::op_rf_mult(p, m, " + at + ", prog, c.q0, c.kill_cavity());");
+                break;
             default: return false;
             }
             pc += 1 + n;

@@ -681,6 +681,56 @@ XT_HD_COLD R rf_kick_ptau(const R zeta, const R ptau_in, const R beta0, const R 
     return ptau;
 }
 
+template <class R>
+struct RFMOut { R px, py, ptau; };
+
+/* track_rf.h:18-167 for an RF multipole (harmonic 0, absolute_time 0, no
+   transverse voltage): the transverse kicks of orders 0..order and the new
+   ptau after the energy kick (unchanged when kill_energy_kick).  The
+   element-uniform terms come from the descriptor d (RFD_* in
+   xt_program.h).  Out of line, scalar arguments. */
+template <class R>
+XT_HD_COLD RFMOut<R> rf_multipole_kick(const R x, const R y, const R px, const R py,
+                                       const R zeta, const R ptau_in, const R beta0,
+                                       const R charge_ratio, const R chi, const R p0c,
+                                       const double voltage, const double frequency,
+                                       const double lag, const double phase,
+                                       const double *__restrict__ d, const double q0,
+                                       const bool kill_energy_kick) {
+    const R phase0 = R(0.0);
+    const R q = xfabs(q0) * charge_ratio;
+    const R tau = zeta / beta0;
+    const R energy_kick =
+        q * voltage * xsin(phase0 + DEG2RAD * lag + phase - (2.0 * PI) / C_LIGHT * frequency * tau);
+    const int64_t order = wbits(d[RFD_ORDER]);
+    R dpx = R(0.0), dpy = R(0.0), dptr = R(0.0), zre = R(1.0), zim = R(0.0);
+    for (int64_t kk = 0; kk <= order; kk++) {
+        const double *t = d + RFD_TERMS + RFD_STRIDE * kk;
+        const R pn_kk = phase0 + t[0] + t[1] - (2.0 * PI) / C_LIGHT * frequency * tau;
+        const R ps_kk = phase0 + t[2] + t[3] - (2.0 * PI) / C_LIGHT * frequency * tau;
+        const double bal_n_kk = t[4], bal_s_kk = t[5];
+        R sn, cn, ss, cs;
+        xsincos(pn_kk, &sn, &cn);
+        xsincos(ps_kk, &ss, &cs);
+        dpx = dpx + (cn * (bal_n_kk * zre) - cs * (bal_s_kk * zim));
+        dpy = dpy + (cs * (bal_s_kk * zre) + cn * (bal_n_kk * zim));
+        const R zret = zre * x - zim * y;
+        zim = zim * x + zre * y;
+        zre = zret;
+        dptr = dptr + (sn * (bal_n_kk * zre) - ss * (bal_s_kk * zim));
+    }
+    const R rfmultipole_energy_kick = -q * ((frequency * (2.0 * PI / C_LIGHT) * p0c) * dptr);
+    RFMOut<R> o;
+    o.px = px + (-chi * dpx);
+    o.py = py + chi * dpy;
+    o.ptau = ptau_in;
+    if (!kill_energy_kick) {
+        const R mass_ratio = charge_ratio / chi;
+        o.ptau = ptau_in + (energy_kick + rfmultipole_energy_kick) / p0c / mass_ratio;
+    }
+    return o;
+}
+
 /* ---- element ops, shared by the interpreter and generated line code ------- */
 
 /* track_magnet_drift.h:500-509: four polar drifts with the k0 kicks of the
@@ -745,6 +795,20 @@ XT_HD void op_rf_kick(PState<R> &p, const PRef &m, const double *__restrict__ a,
                              R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), p.at_turn, a,
                              line_length, q0, t_sim),
                 m);
+}
+
+/* rfmultipole.h:11-47: one kick (model -1); the energy update with
+   pz_only = 1 (custom_api.h:196-216) */
+template <class R>
+XT_HD void op_rf_mult(PState<R> &p, const PRef &m, const double *__restrict__ a,
+                      const double *__restrict__ prog, const double q0, const bool kill_cavity) {
+    const RFMOut<R> o = rf_multipole_kick(
+        p.x, p.y, p.px, p.py, p.zeta, R(m.P->ptau[m.i]), R(m.P->beta0[m.i]),
+        R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), a[0], a[1], a[2], a[3],
+        prog + wbits(a[4]), q0, kill_cavity);
+    p.px = o.px;
+    p.py = o.py;
+    if (!kill_cavity) update_ptau(p, o.ptau, m);
 }
 
 /* limitrect.h:10-38 */
@@ -962,6 +1026,10 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
         case OP_S_ROT:
             op_s_rot(p, a[0], a[1]);
             pc += 3;
+            break;
+        case OP_RF_MULT:
+            op_rf_mult(p, m, a, prog, c.q0, c.kill_cavity());
+            pc += 6;
             break;
         default:
             p.state = -999;   /* corrupt program: never produced by the builder */

@@ -42,12 +42,22 @@ enum xt_op {
     OP_X_ROT = 17,         /* sin cos tan as XRotation_single_particle takes   */
     OP_Y_ROT = 18,         /* sin cos tan as YRotation_single_particle takes   */
     OP_S_ROT = 19,         /* sin cos                                          */
-    OP_NUM = 20
+    OP_RF_MULT = 20,       /* V freq lag phase desc  (RF multipole kick)       */
+    OP_NUM = 21
 };
 
 /* number of constants following each op word */
 #define XT_NARGS_KICK 6
-#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2, 1, 3, 3, 2}
+#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2, 1, 3, 3, 2, 5}
+
+/* RF multipole descriptor, referenced by OP_RF_MULT through an absolute
+ * offset into prog: the order (bits), then for k = 0..order the six
+ * element-uniform terms of track_rf.h:96-118: DEG2RAD*pn[k], phase_n[k],
+ * DEG2RAD*ps[k], phase_s[k], factor*knl[k]/k!, factor*ksl[k]/k! (factor =
+ * +1, or -1 when backtracking). */
+#define RFD_ORDER 0
+#define RFD_TERMS 1
+#define RFD_STRIDE 6
 
 /* Multipole-kick descriptor (track_magnet_kick.h:24-144), referenced by
  * OP_KICK through an absolute offset into prog:

@@ -9,6 +9,7 @@ parameter derivation of the reference classes:
   Cavity        cavity.py:21-171
   LimitRect     limit_rect.py, LimitEllipse limit_ellipse.py:12-80
   DipoleEdge    dipole_edge.py:12-150
+  RFMultipole   rf_multipole.py:11-147, _common.py:419-541
   Marker        marker.py
 
 Each element serialises itself into the raw element table read by the HIP
@@ -27,7 +28,7 @@ from .constants import UNLIMITED
 # element type ids (include/xtrack_amd.h)
 T_MARKER, T_DRIFT, T_MULTIPOLE, T_QUADRUPOLE, T_BEND = 0, 1, 2, 3, 4
 T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, T_LIMITRECT, T_LIMITELLIPSE = 5, 6, 7, 8, 9
-T_DIPOLEEDGE, T_TRANSFORMED = 10, 11
+T_DIPOLEEDGE, T_TRANSFORMED, T_RFMULTIPOLE = 10, 11, 12
 
 DEFAULT_MULTIPOLE_ORDER = 5          # _common.py:18
 
@@ -203,6 +204,7 @@ class _Magnet(BeamElement):
         order = kwargs.pop('order', None)
         knl = kwargs.pop('knl', None)
         ksl = kwargs.pop('ksl', None)
+        kwargs.pop('inv_factorial_order', None)   # derived from order (_common.py:538-539)
         if self._default_order is not None:
             order = order or self._default_order
         self._order, self.knl, self.ksl, self.inv_factorial_order = \
@@ -595,9 +597,50 @@ class DipoleEdge(BeamElement):
         return T_DIPOLEEDGE, [self.r21, self.r43, float(self._model), self.delta_taper]
 
 
+class RFMultipole(BeamElement):
+    """Thin multipole whose strengths and energy kick are modulated with the
+    arrival time (rf_multipole.py:11-147).  The arrays knl, ksl, pn, ps
+    (degrees, deprecated), phase_n, phase_s share one length, at least
+    order + 1 with order defaulting to 5 (_common.py:419-541)."""
+    _type = T_RFMULTIPOLE
+    _ARRAYS = ('knl', 'ksl', 'pn', 'ps', 'phase_n', 'phase_s')
+
+    def __init__(self, **kwargs):
+        self._init_misalignment(kwargs)
+        self.voltage = float(kwargs.pop('voltage', 0.0))
+        self.frequency = float(kwargs.pop('frequency', 0.0))
+        self.lag = float(kwargs.pop('lag', 0.0))
+        self.phase = float(kwargs.pop('phase', 0.0))
+        self.absolute_time = int(kwargs.pop('absolute_time', 0))
+        order = kwargs.pop('order', None)
+        kwargs.pop('inv_factorial_order', None)
+        arrays = {nn: kwargs.pop(nn, None) for nn in self._ARRAYS}
+        if kwargs:
+            raise NameError(f'Invalid argument(s): {set(kwargs)}')
+        order = order or DEFAULT_MULTIPOLE_ORDER
+        lengths = [len(v) if v is not None else 0 for v in arrays.values()]
+        target_len = max(order + 1, *lengths)
+        for nn, vv in arrays.items():
+            arr = np.zeros(target_len, dtype=np.float64)
+            if vv is not None:
+                arr[:len(vv)] = np.asarray(vv, dtype=np.float64)
+            setattr(self, nn, arr)
+        self._order = target_len - 1
+        self.inv_factorial_order = 1.0 / math.factorial(self._order)
+
+    @property
+    def order(self):
+        return self._order
+
+    def raw(self):
+        head = [self.voltage, self.frequency, self.lag, self.phase, float(self._order)]
+        return T_RFMULTIPOLE, head + [float(v) for nn in self._ARRAYS
+                                      for v in getattr(self, nn)]
+
+
 ELEMENT_CLASSES = {c.__name__: c for c in (Marker, Drift, Multipole, Quadrupole, Bend,
                                            Sextupole, Octupole, Cavity, LimitRect,
-                                           LimitEllipse, DipoleEdge)}
+                                           LimitEllipse, DipoleEdge, RFMultipole)}
 
 # keys of serialised elements that carry no tracking parameter (naming and
 # aperture bookkeeping of the reference's environment)
