@@ -240,11 +240,10 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
-         "    // generated code for chi == 1 particles, the interpreter for the others\n"
-         "    xt::track_particle_with<false>(P, i, xt_gen::Runner{rec, entry, prog},\n"
-         "                            xt::ProgramRunner{entry, prog},\n"
-         "                            c, num_turns, ele_start, num_ele_track,\n"
-         "                            flag_end_turn_actions, flag_reset_s, flag_monitor, M);\n"
+         "    // chi == 1 particles; the interpreter kernel tracks the others\n"
+         "    xt::track_particle_chi1(P, i, xt_gen::Runner{rec, entry, prog}, c, num_turns,\n"
+         "                            ele_start, num_ele_track, flag_end_turn_actions,\n"
+         "                            flag_reset_s, flag_monitor, M);\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over all particles\n"

@@ -58,9 +58,11 @@ __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     const int64_t *__restrict__ entry, const double *__restrict__ prog, double line_length,
     int64_t num_ele_line, xtm_particles P, int32_t num_turns, int32_t ele_start,
     int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,
-    int32_t flag_monitor, xtm_monitor M, uint64_t flags) {
+    int32_t flag_monitor, xtm_monitor M, uint64_t flags, int32_t skip_chi1) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= P.capacity) return;
+    // after the compiled line kernel: only the particles it left (chi != 1)
+    if (skip_chi1 && P.chi[i] == 1.0) return;
     xt::track_particle(P, i, entry, prog, line_length, num_ele_line, num_turns, ele_start,
                        num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor, M,
                        flags);
@@ -389,6 +391,14 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
         e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
                                   (hipStream_t)stream, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
+        /* particles with chi != 1 on the interpreter, same stream */
+        hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
+                           (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length,
+                           (int64_t)L->num_elements, *P, num_turns, ele_start, num_ele_track,
+                           flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor, M,
+                           track_flags, (int32_t)1);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
         return XTM_OK;
     }
     /* backtracking always runs on the interpreter, with the backtracking
@@ -397,7 +407,7 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
                        (hipStream_t)stream, backtrack ? L->d_entry_bt : L->d_entry, L->d_prog,
                        L->line_length, (int64_t)L->num_elements, *P,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
-                       flag_reset_s_at_end_turn, flag_monitor, M, track_flags);
+                       flag_reset_s_at_end_turn, flag_monitor, M, track_flags, (int32_t)0);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
     return XTM_OK;

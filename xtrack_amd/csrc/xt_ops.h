@@ -1221,6 +1221,19 @@ struct ProgramRunner {
    run_other all others.  With ALLOW_BT, c.backtrack() selects the
    backtracking loop (run_other, which must then execute the backtracking
    programs); without, the caller guarantees forward tracking. */
+/* The loops for particles with chi != 1 and for backtracking, out of
+   line: their code (the general interpreter) then takes no part in the
+   register allocation of the kernel's chi == 1 loop. */
+template <bool BT, class Runner>
+XT_HD_COLD void track_particle_loop_cold(const xtm_particles &P, int64_t i, const Runner &run,
+                                         const TrackCtx &c, int32_t num_turns,
+                                         int32_t ele_start, int32_t num_ele_track,
+                                         int32_t flag_end_turn_actions, int32_t flag_reset_s,
+                                         int32_t flag_monitor, const xtm_monitor &M) {
+    track_particle_loop<false, BT>(P, i, run, c, num_turns, ele_start, num_ele_track,
+                                   flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+}
+
 template <bool ALLOW_BT, class Runner1, class RunnerOther>
 XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner1 &run_chi1,
                                const RunnerOther &run_other, const TrackCtx &c,
@@ -1229,17 +1242,31 @@ XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner1 
                                int32_t flag_monitor, const xtm_monitor &M) {
     if (!(P.state[i] > 0)) return;
     if (ALLOW_BT && c.backtrack())
-        track_particle_loop<false, true>(P, i, run_other, c, num_turns, ele_start,
-                                         num_ele_track, flag_end_turn_actions, flag_reset_s,
-                                         flag_monitor, M);
+        track_particle_loop_cold<true>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
+                                       flag_end_turn_actions, flag_reset_s, flag_monitor, M);
     else if (P.chi[i] == 1.0)
         track_particle_loop<true, false>(P, i, run_chi1, c, num_turns, ele_start,
                                          num_ele_track, flag_end_turn_actions, flag_reset_s,
                                          flag_monitor, M);
     else
-        track_particle_loop<false, false>(P, i, run_other, c, num_turns, ele_start,
-                                          num_ele_track, flag_end_turn_actions, flag_reset_s,
-                                          flag_monitor, M);
+        track_particle_loop_cold<false>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
+                                        flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+}
+
+/* The compiled line kernel's body: forward tracking of the particles with
+   chi == 1 only; the interpreter kernel, launched after it on the same
+   stream with skip_chi1 set, tracks the others.  Keeping the interpreter
+   out of this kernel keeps its register allocation and its scratch frame
+   to those of the generated code. */
+template <class Runner1>
+XT_HD void track_particle_chi1(const xtm_particles &P, int64_t i, const Runner1 &run_chi1,
+                               const TrackCtx &c, int32_t num_turns, int32_t ele_start,
+                               int32_t num_ele_track, int32_t flag_end_turn_actions,
+                               int32_t flag_reset_s, int32_t flag_monitor,
+                               const xtm_monitor &M) {
+    if (!(P.state[i] > 0) || !(P.chi[i] == 1.0)) return;
+    track_particle_loop<true, false>(P, i, run_chi1, c, num_turns, ele_start, num_ele_track,
+                                     flag_end_turn_actions, flag_reset_s, flag_monitor, M);
 }
 
 /* entry: the forward or, when flags hold XS_FLAG_BACKTRACK, the
