@@ -22,7 +22,7 @@ extern "C" int32_t xth_track_line(const int32_t *types, const int64_t *offsets,
     int st = xt_build_program(types, offsets, params, n, entry, prog, err, &entry_bt);
     if (st) return st;
     xtm_monitor m0 = {};
-    const xtm_monitor &MM = M ? *M : m0;
+    const xt::MonRef MM{M ? M : &m0, P};
     const int64_t *ent = (flags & 1) ? entry_bt.data() : entry.data();
     for (int64_t i = 0; i < P->capacity; i++)
         xt::track_particle(*P, i, ent, prog.data(), line_length, n, num_turns, ele_start,

@@ -236,14 +236,15 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    const double *__restrict__ prog, double line_length, xtm_particles P,\n"
          "    int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
          "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
-         "    xtm_monitor M, uint64_t flags) {\n"
+         "    const xtm_monitor *__restrict__ Md, const xtm_particles *__restrict__ Pd,\n"
+         "    uint64_t flags) {\n"
          "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
          "    // chi == 1 particles; the interpreter kernel tracks the others\n"
          "    xt::track_particle_chi1(P, i, xt_gen::Runner{rec, entry, prog}, c, num_turns,\n"
          "                            ele_start, num_ele_track, flag_end_turn_actions,\n"
-         "                            flag_reset_s, flag_monitor, M);\n"
+         "                            flag_reset_s, flag_monitor, xt::MonRef{Md, Pd});\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over all particles\n"
@@ -253,11 +254,12 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    int32_t flag_monitor, const xtm_monitor *M, uint64_t flags) {\n"
          "    xtm_monitor m0 = {};\n"
          "    const xt::TrackCtx c{flags, line_length, P->q0, P->t_sim};\n"
+         "    const xt::MonRef mon{M ? M : &m0, P};\n"
          "    for (int64_t i = 0; i < P->capacity; i++)\n"
          "        xt::track_particle_with<false>(*P, i, xt_gen::Runner{rec, entry, prog},\n"
          "                                xt::ProgramRunner{entry, prog}, c, num_turns,\n"
          "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
-         "                                flag_reset_s, flag_monitor, M ? *M : m0);\n"
+         "                                flag_reset_s, flag_monitor, mon);\n"
          "}\n"
          "#endif\n";
     src = std::move(s);

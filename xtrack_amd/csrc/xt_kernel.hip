@@ -54,18 +54,34 @@ constexpr int BLOCK = 256;
 #define XTM_OCCUPANCY
 #endif
 
+/* The particles struct is read through Pd, a device copy, so that passing
+   it by reference to the out-of-line loops copies nothing into the waves'
+   scratch frames. */
 __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     const int64_t *__restrict__ entry, const double *__restrict__ prog, double line_length,
-    int64_t num_ele_line, xtm_particles P, int32_t num_turns, int32_t ele_start,
-    int32_t num_ele_track, int32_t flag_end_turn_actions, int32_t flag_reset_s,
-    int32_t flag_monitor, xtm_monitor M, uint64_t flags, int32_t skip_chi1) {
+    int64_t num_ele_line, const xtm_particles *__restrict__ Pd, int32_t num_turns,
+    int32_t ele_start, int32_t num_ele_track, int32_t flag_end_turn_actions,
+    int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor *__restrict__ Md,
+    uint64_t flags, int32_t skip_chi1) {
+    const xtm_particles &P = *Pd;
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= P.capacity) return;
     // after the compiled line kernel: only the particles it left (chi != 1)
     if (skip_chi1 && P.chi[i] == 1.0) return;
     xt::track_particle(P, i, entry, prog, line_length, num_ele_line, num_turns, ele_start,
-                       num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor, M,
-                       flags);
+                       num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor,
+                       xt::MonRef{Md, Pd}, flags);
+}
+
+/* device copies of the particle struct and the monitor for the tracking
+   kernels (stream-ordered before them; kernel arguments are captured at
+   launch, so the caller's structs need not outlive the call) */
+__global__ void args_kernel(xtm_particles P, xtm_monitor M, xtm_particles *Pd,
+                            xtm_monitor *Md) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *Pd = P;
+        *Md = M;
+    }
 }
 
 __global__ void math_kernel(int32_t fn, const double *__restrict__ x, double *__restrict__ y,
@@ -114,6 +130,10 @@ struct xtm_line {
     std::vector<int64_t> entry;   // host copies (operation counting, code generation)
     std::vector<int64_t> entry_bt;
     std::vector<double> prog;
+    // device copies of the particle and monitor structs read by monitor
+    // records (written by monitor_args_kernel before a launch with a monitor)
+    xtm_particles *d_mon_p = nullptr;
+    xtm_monitor *d_mon_m = nullptr;
     // generated line code (xtm_line_specialize)
     int32_t *d_rec = nullptr;
     hipModule_t module = nullptr;
@@ -233,7 +253,9 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
     const size_t ne = entry.empty() ? 1 : entry.size();
     if ((e = hipMalloc(&L->d_entry, ne * sizeof(int64_t))) != hipSuccess ||
         (e = hipMalloc(&L->d_entry_bt, ne * sizeof(int64_t))) != hipSuccess ||
-        (e = hipMalloc(&L->d_prog, prog.size() * sizeof(double))) != hipSuccess) {
+        (e = hipMalloc(&L->d_prog, prog.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&L->d_mon_p, sizeof(xtm_particles))) != hipSuccess ||
+        (e = hipMalloc(&L->d_mon_m, sizeof(xtm_monitor))) != hipSuccess) {
         xtm_line_destroy(L);
         return hip_fail(e, "hipMalloc");
     }
@@ -264,6 +286,8 @@ int32_t xtm_line_destroy(xtm_line *L) {
     if (L->d_entry_bt) (void)hipFree(L->d_entry_bt);
     if (L->d_prog) (void)hipFree(L->d_prog);
     if (L->d_rec) (void)hipFree(L->d_rec);
+    if (L->d_mon_p) (void)hipFree(L->d_mon_p);
+    if (L->d_mon_m) (void)hipFree(L->d_mon_m);
     if (L->module) (void)hipModuleUnload(L->module);
     delete L;
     return XTM_OK;
@@ -373,6 +397,12 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
     xtm_monitor M;
     if (flag_monitor) M = *monitor;
     else std::memset(&M, 0, sizeof(M));
+    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, *P, M,
+                       L->d_mon_p, L->d_mon_m);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "args_kernel launch");
+    const xtm_monitor *Md = L->d_mon_m;
+    const xtm_particles *Pd = L->d_mon_p;
     const int64_t nblocks = (P->capacity + BLOCK - 1) / BLOCK;
     if (nblocks > 0x7fffffff) return fail(XTM_ERR_ARG, "too many particles");
     const bool backtrack = (track_flags >> XTM_FLAG_BACKTRACK) & 1;
@@ -387,15 +417,15 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
         int32_t a_mon = flag_monitor;
         uint64_t a_flags = track_flags;
         void *args[] = {&rec, &entry, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
-                        &a_end, &a_reset, &a_mon, &M, &a_flags};
+                        &a_end, &a_reset, &a_mon, &Md, &Pd, &a_flags};
         e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
                                   (hipStream_t)stream, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
         /* particles with chi != 1 on the interpreter, same stream */
         hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
                            (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length,
-                           (int64_t)L->num_elements, *P, num_turns, ele_start, num_ele_track,
-                           flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor, M,
+                           (int64_t)L->num_elements, Pd, num_turns, ele_start, num_ele_track,
+                           flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor, Md,
                            track_flags, (int32_t)1);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
@@ -405,9 +435,9 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
        programs */
     hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
                        (hipStream_t)stream, backtrack ? L->d_entry_bt : L->d_entry, L->d_prog,
-                       L->line_length, (int64_t)L->num_elements, *P,
+                       L->line_length, (int64_t)L->num_elements, Pd,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
-                       flag_reset_s_at_end_turn, flag_monitor, M, track_flags, (int32_t)0);
+                       flag_reset_s_at_end_turn, flag_monitor, Md, track_flags, (int32_t)0);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
     return XTM_OK;

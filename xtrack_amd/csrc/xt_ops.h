@@ -1112,9 +1112,18 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
     }
 }
 
+/* What the monitor records need: the monitor and the tracked particles'
+   struct, by pointer.  On the GPU both point to device copies, so that no
+   kernel argument has its address taken (which would copy it into every
+   wave's scratch frame). */
+struct MonRef {
+    const xtm_monitor *M;
+    const xtm_particles *P;
+};
+
 #define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
-    monitor_record(M, P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta, p.rpp, p.rvv, p.s,      \
-                   p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
+    monitor_record(*mon.M, *mon.P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta, p.rpp,       \
+                   p.rvv, p.s, p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
 
 /* ---- one particle through [ele_start, ele_start+num_ele_track) for
    num_turns turns: the per-thread body of track_line (tracker.py:596-734)
@@ -1133,7 +1142,7 @@ template <bool CHI1, bool BT, class Runner>
 XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &run,
                                const TrackCtx &c, int32_t num_turns, int32_t ele_start,
                                int32_t num_ele_track, int32_t flag_end_turn_actions,
-                               int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor &M) {
+                               int32_t flag_reset_s, int32_t flag_monitor, const MonRef &mon) {
     PState<double> p;
     p.x = P.x[i]; p.px = P.px[i]; p.y = P.y[i]; p.py = P.py[i];
     p.zeta = P.zeta[i]; p.delta = P.delta[i];
@@ -1229,9 +1238,9 @@ XT_HD_COLD void track_particle_loop_cold(const xtm_particles &P, int64_t i, cons
                                          const TrackCtx &c, int32_t num_turns,
                                          int32_t ele_start, int32_t num_ele_track,
                                          int32_t flag_end_turn_actions, int32_t flag_reset_s,
-                                         int32_t flag_monitor, const xtm_monitor &M) {
+                                         int32_t flag_monitor, const MonRef &mon) {
     track_particle_loop<false, BT>(P, i, run, c, num_turns, ele_start, num_ele_track,
-                                   flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+                                   flag_end_turn_actions, flag_reset_s, flag_monitor, mon);
 }
 
 template <bool ALLOW_BT, class Runner1, class RunnerOther>
@@ -1239,18 +1248,18 @@ XT_HD void track_particle_with(const xtm_particles &P, int64_t i, const Runner1 
                                const RunnerOther &run_other, const TrackCtx &c,
                                int32_t num_turns, int32_t ele_start, int32_t num_ele_track,
                                int32_t flag_end_turn_actions, int32_t flag_reset_s,
-                               int32_t flag_monitor, const xtm_monitor &M) {
+                               int32_t flag_monitor, const MonRef &mon) {
     if (!(P.state[i] > 0)) return;
     if (ALLOW_BT && c.backtrack())
         track_particle_loop_cold<true>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
-                                       flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+                                       flag_end_turn_actions, flag_reset_s, flag_monitor, mon);
     else if (P.chi[i] == 1.0)
         track_particle_loop<true, false>(P, i, run_chi1, c, num_turns, ele_start,
                                          num_ele_track, flag_end_turn_actions, flag_reset_s,
-                                         flag_monitor, M);
+                                         flag_monitor, mon);
     else
         track_particle_loop_cold<false>(P, i, run_other, c, num_turns, ele_start, num_ele_track,
-                                        flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+                                        flag_end_turn_actions, flag_reset_s, flag_monitor, mon);
 }
 
 /* The compiled line kernel's body: forward tracking of the particles with
@@ -1263,10 +1272,10 @@ XT_HD void track_particle_chi1(const xtm_particles &P, int64_t i, const Runner1 
                                const TrackCtx &c, int32_t num_turns, int32_t ele_start,
                                int32_t num_ele_track, int32_t flag_end_turn_actions,
                                int32_t flag_reset_s, int32_t flag_monitor,
-                               const xtm_monitor &M) {
+                               const MonRef &mon) {
     if (!(P.state[i] > 0) || !(P.chi[i] == 1.0)) return;
     track_particle_loop<true, false>(P, i, run_chi1, c, num_turns, ele_start, num_ele_track,
-                                     flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+                                     flag_end_turn_actions, flag_reset_s, flag_monitor, mon);
 }
 
 /* entry: the forward or, when flags hold XS_FLAG_BACKTRACK, the
@@ -1275,12 +1284,12 @@ XT_HD void track_particle(const xtm_particles &P, int64_t i, const int64_t *__re
                           const double *__restrict__ prog, double line_length,
                           int64_t num_ele_line, int32_t num_turns, int32_t ele_start,
                           int32_t num_ele_track, int32_t flag_end_turn_actions,
-                          int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor &M,
+                          int32_t flag_reset_s, int32_t flag_monitor, const MonRef &mon,
                           uint64_t flags) {
     const TrackCtx c{flags, line_length, P.q0, P.t_sim, num_ele_line};
     const ProgramRunner run{entry, prog};
     track_particle_with<true>(P, i, run, run, c, num_turns, ele_start, num_ele_track,
-                              flag_end_turn_actions, flag_reset_s, flag_monitor, M);
+                              flag_end_turn_actions, flag_reset_s, flag_monitor, mon);
 }
 
 #undef XT_MONITOR_RECORD
