@@ -278,3 +278,25 @@ def test_hllhc_lattice_bit_exact():
     bad = compare_exact(pg, pc)
     assert (pc.sorted_by_id()['state'] <= 0).sum() > 100
     assert not bad, bad
+
+
+def test_twiss_through_gpu_tracker():
+    """Twiss (closed orbit, one-turn matrix, normal form, element-by-element
+    propagation) with the GPU tracker equals the one with the oracle: the
+    probes are tracked bit for bit."""
+    from xtrack_amd import lattices
+    print('[twiss]', flush=True)
+
+    def mk():
+        return lattices.collider_ring(n_arcs=1, n_cells=4, k2f=0.03, k2d=-0.06)
+
+    lg, lc = mk(), mk()
+    lg.build_tracker(device='cuda:0', compile=_MODE['compile'])
+    lc.build_tracker(backend=OracleBackend())
+    for method in ('4d', '6d'):
+        tg, tc = lg.twiss(method=method), lc.twiss(method=method)
+        for nn in ('betx', 'bety', 'alfx', 'alfy', 'mux', 'muy', 'dx', 'dpx', 'x', 'px',
+                   'zeta', 'pzeta'):
+            assert np.array_equal(tg[nn], tc[nn]), (method, nn)
+        assert tg.qx == tc.qx and tg.qy == tc.qy and tg.dqx == tc.dqx
+        assert np.array_equal(tg.R_matrix, tc.R_matrix)

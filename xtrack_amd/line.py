@@ -87,6 +87,12 @@ class Line:
                                    backtrack=backtrack, time=time,
                                    with_progress=with_progress)
 
+    def twiss(self, particle_ref=None, method='6d', delta0=0.0, **kwargs):
+        """Linear optics of the ring from its tracker (xtrack line.py
+        Line.twiss); see xtrack_amd.twiss.twiss."""
+        from .twiss import twiss as _twiss
+        return _twiss(self, particle_ref=particle_ref, method=method, delta0=delta0, **kwargs)
+
     def build_particles(self, **kwargs):
         """Particles with the reference quantities of ``particle_ref`` and the
         given physical coordinates."""
