@@ -48,7 +48,8 @@ extern "C" {
 #define XTM_DIPOLEEDGE 10
 #define XTM_TRANSFORMED 11
 #define XTM_RFMULTIPOLE 12
-#define XTM_NUM_TYPES 13
+#define XTM_SLICE 13
+#define XTM_NUM_TYPES 14
 
 /* ---- raw parameter layout ---------------------------------------------------
  * Element i occupies params[offsets[i] ...].  Integer fields are stored as
@@ -144,6 +145,25 @@ extern "C" {
 #define XTM_RFM_ORDER 4
 #define XTM_RFM_ARRAYS 5
 
+/* SLICE: a slice of a thick element made by Line.slice_thick_elements
+ * (beam_elements/slice_elements_{thin,thick,drift,edge}.py and the
+ * elements_src/<kind>_slice_<class>.h): the slice weight, the kind (XTM_SLICE_*), the
+ * parent's type, then the parent's record (without its misalignment,
+ * which wraps the slice instead).  Thin: the parent's magnet kick only
+ * (model -1) with the weight; thick: the parent's body with the weight;
+ * drift: an expanded drift of weight x parent length; entry/exit: the
+ * parent's entry/exit edge only.  Parents: Multipole (thick), Quadrupole,
+ * Bend, Sextupole, Octupole; drift slices also Drift. */
+#define XTM_SL_WEIGHT 0
+#define XTM_SL_KIND 1
+#define XTM_SL_PARENT_TYPE 2
+#define XTM_SL_PARENT 3
+#define XTM_SLICE_THIN 0
+#define XTM_SLICE_THICK 1
+#define XTM_SLICE_DRIFT 2
+#define XTM_SLICE_ENTRY 3
+#define XTM_SLICE_EXIT 4
+
 /* TRANSFORMED: an element with non-zero misalignment fields, wrapped as in
  * track_local_particle_with_transformations.h:88-168 with the frame
  * transforms of track_misalignments.h (straight :38-112, curved :115-386).
@@ -152,7 +172,8 @@ extern "C" {
  * classes that are not thick), angle and h; 1 for classes that curve the
  * reference frame (Bend), else 0; the inner type; then the inner element's
  * record.  Inner types: Multipole, Quadrupole, Bend, Sextupole, Octupole,
- * Cavity, LimitRect, LimitEllipse, DipoleEdge, RFMultipole. */
+ * Cavity, LimitRect, LimitEllipse, DipoleEdge, RFMultipole, and slices
+ * (thin, thick and edge slices take their parent's misalignment). */
 #define XTM_TR_SHIFT_X 0
 #define XTM_TR_SHIFT_Y 1
 #define XTM_TR_SHIFT_S 2

@@ -631,7 +631,19 @@ static void magnet_edge(LP *part0, int64_t model, int is_exit, double half_gap,
  * Magnet elements: per-class argument mapping (multipole.h:16-69,
  * quadrupole.h:15-75, bend.h:15-76, sextupole.h / octupole.h:15-75) then
  * track_magnet_particles (track_magnet.h:289-648, no rbend, no radiation). */
+/* track_magnet_particles (track_magnet.h:286-560) for a whole element
+   (slice_kind -1) or, as the slice elements call it, a thin slice (model
+   -1, edges off), a thick slice (edges off) or an entry / exit edge slice
+   (body off, the other edge off) with the slice weight. */
+static int magnet_track_weighted(LP *part0, int type, const double *p, double weight,
+                                 int slice_kind);
+
 static int magnet_track(LP *part0, int type, const double *p) {
+    return magnet_track_weighted(part0, type, p, 1., -1);
+}
+
+static int magnet_track_weighted(LP *part0, int type, const double *p, double weight,
+                                 int slice_kind) {
     const int64_t order = (int64_t)p[XTM_M_ORDER];
     const int64_t nrel = (int64_t)p[XTM_M_NREL];
     const double *knl = p + XTM_M_ARRAYS;
@@ -695,7 +707,17 @@ static int magnet_track(LP *part0, int type, const double *p) {
     default:
         return XTM_ERR_ARG;
     }
-    const double weight = 1.;
+    int body_active = 1;
+    if (slice_kind == XTM_SLICE_THIN) {
+        model = -1;
+        edge_entry_active = 0; edge_exit_active = 0;
+    } else if (slice_kind == XTM_SLICE_THICK) {
+        edge_entry_active = 0; edge_exit_active = 0;
+    } else if (slice_kind == XTM_SLICE_ENTRY) {
+        body_active = 0; edge_exit_active = 0;
+    } else if (slice_kind == XTM_SLICE_EXIT) {
+        body_active = 0; edge_entry_active = 0;
+    }
     double const inv_factorial_order = p[XTM_M_INV_FACTORIAL_ORDER];
     const int64_t order_rel = nrel - 1;
     const double inv_factorial_order_rel = one_over_factorial(nrel - 1);
@@ -734,9 +756,12 @@ static int magnet_track(LP *part0, int type, const double *p) {
                     factor_backtrack_edge);
     }
     /* body */
+    if (!body_active) goto exit_edge;
     if (integrator == 0) integrator = default_integrator;
     if (model == 0) model = default_model;
     if (model == -1) { integrator = 3; num_multipole_kicks = 1; }
+    if (weight != 1.0 && num_multipole_kicks > 0)
+        num_multipole_kicks = (int64_t)ceil(num_multipole_kicks * weight);
     if (num_multipole_kicks == 0) {
         if (!kick_is_inactive(order, knl, ksl, k0, k1, k2, k3, k0s, k1s, k2s, k3s, h)) {
             if (fabs(h) < 1e-8) {
@@ -765,6 +790,7 @@ static int magnet_track(LP *part0, int type, const double *p) {
     FOR_ACTIVE(part0, part) {
         magnet_body(part, &a);
     } END_FOR_ACTIVE
+exit_edge:
     if (edge_exit_active) {
         if (!(edge_exit_model == 0 || edge_exit_model == -1 || edge_exit_model == 3))
             return XTM_ERR_UNSUPPORTED;
@@ -1440,7 +1466,8 @@ static int transformed_track(LP *part0, const double *p) {
     int const curved = p[XTM_TR_CURVED] != 0.0;
     int const inner = (int)p[XTM_TR_INNER_TYPE];
     const double *q = p + XTM_TR_INNER;
-    if (!((inner >= XTM_MULTIPOLE && inner <= XTM_DIPOLEEDGE) || inner == XTM_RFMULTIPOLE))
+    if (!((inner >= XTM_MULTIPOLE && inner <= XTM_DIPOLEEDGE) || inner == XTM_RFMULTIPOLE ||
+          inner == XTM_SLICE))
         return XTM_ERR_UNSUPPORTED;
     int const backtrack = check_flag(part0, XTM_FLAG_BACKTRACK);
 
@@ -1481,6 +1508,29 @@ static int transformed_track(LP *part0, const double *p) {
     return st;
 }
 
+/* Slice elements (slice_elements_*.py, elements_src/<kind>_slice_<class>.h): drift
+ * slices Drift_single_particle over weight x parent length (negated when
+ * backtracking); the others the parent's magnet with the slice's weight. */
+static int slice_track(LP *part0, const double *p) {
+    double const weight = p[XTM_SL_WEIGHT];
+    int const kind = (int)p[XTM_SL_KIND];
+    int const parent = (int)p[XTM_SL_PARENT_TYPE];
+    const double *q = p + XTM_SL_PARENT;
+    if (kind == XTM_SLICE_DRIFT) {
+        double plen;
+        if (parent == XTM_DRIFT) plen = q[XTM_D_LENGTH];
+        else if (parent >= XTM_MULTIPOLE && parent <= XTM_OCTUPOLE) plen = q[XTM_M_LENGTH];
+        else return XTM_ERR_UNSUPPORTED;
+        double length;
+        if (check_flag(part0, XTM_FLAG_BACKTRACK)) length = -weight * plen;
+        else length = weight * plen;
+        FOR_ACTIVE(part0, part) { drift_expanded(part, length); } END_FOR_ACTIVE
+        return XTM_OK;
+    }
+    if (!(parent >= XTM_MULTIPOLE && parent <= XTM_OCTUPOLE)) return XTM_ERR_UNSUPPORTED;
+    return magnet_track_weighted(part0, parent, q, weight, kind);
+}
+
 static int track_element_body(LP *lp, int type, const double *p) {
     int st = XTM_OK;
     switch (type) {
@@ -1494,6 +1544,7 @@ static int track_element_body(LP *lp, int type, const double *p) {
     case XTM_DIPOLEEDGE: st = dipoleedge_track(lp, p); break;
     case XTM_TRANSFORMED: st = transformed_track(lp, p); break;
     case XTM_RFMULTIPOLE: st = rfmultipole_track(lp, p); break;
+    case XTM_SLICE: st = slice_track(lp, p); break;
     default: return XTM_ERR_ARG;
     }
     return st;
@@ -1503,8 +1554,14 @@ static int track_element(LP *lp, int type, const double *p) {
     int st = track_element_body(lp, type, p);
     if (st) return st;
     /* the class's static thickness decides the global check (tracker.py:681) */
-    int const cls = type == XTM_TRANSFORMED ? (int)p[XTM_TR_INNER_TYPE] : type;
-    if (is_static_thick(cls)) global_aperture_check(lp);
+    int cls = type;
+    const double *q = p;
+    if (type == XTM_TRANSFORMED) { cls = (int)p[XTM_TR_INNER_TYPE]; q = p + XTM_TR_INNER; }
+    /* slice classes: thick and drift slices declare isthick = True */
+    int const thick = cls == XTM_SLICE
+        ? ((int)q[XTM_SL_KIND] == XTM_SLICE_THICK || (int)q[XTM_SL_KIND] == XTM_SLICE_DRIFT)
+        : is_static_thick(cls);
+    if (thick) global_aperture_check(lp);
     return XTM_OK;
 }
 

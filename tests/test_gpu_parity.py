@@ -300,3 +300,17 @@ def test_twiss_through_gpu_tracker():
             assert np.array_equal(tg[nn], tc[nn]), (method, nn)
         assert tg.qx == tc.qx and tg.qy == tc.qy and tg.dqx == tc.dqx
         assert np.array_equal(tg.R_matrix, tc.R_matrix)
+
+
+def test_sliced_line_bit_exact():
+    """Thin, thick, drift and edge slices (misaligned parents included),
+    forward and backtracking."""
+    from test_slicing import sliced_line
+    print('[sliced line]', flush=True)
+    p = random_particles(3000, seed=14, scale=4e-3)
+    for bt in (False, True):
+        lg, lc, pg, pc = _pair(sliced_line, p)
+        lg.track(pg, num_turns=3, backtrack=bt)
+        lc.track(pc, num_turns=3, backtrack=bt)
+        bad = compare_exact(pg, pc)
+        assert not bad, (bt, bad)

@@ -235,3 +235,22 @@ def test_backtrack_sets_flag_for_the_call_only():
     assert rec.calls == [(1, 0, 10, True, 0), (1, 0, 10, True, 0)]
     line.track(xt.Particles(x=[0.0]), num_turns=1)
     assert seen[-1] == 0 and not line.track_flags.XS_FLAG_BACKTRACK
+
+
+@pytest.mark.parametrize('el', [
+    xt.Bend(length=1.0, angle=0.02, k1=0.01, model='rot-kick-rot', edge_entry_model='suppressed'),
+    xt.Bend(length=2.0, angle=0.1, k0=0.04),
+    xt.Multipole(knl=[0, 1], length=0.5, isthick=True),
+    xt.Quadrupole(length=1, k1=0.1, model='drift-kick-drift-exact', shift_x=1e-3),
+    xt.Cavity(voltage=1e6, frequency=4e8, lag=10, model='drift-kick-drift-exact'),
+    xt.DipoleEdge(k=0.01, e1=0.1, hgap=0.02, fint=0.5),
+    xt.Drift(length=1, model='exact'),
+    xt.RFMultipole(knl=[1, 2], frequency=1e8),
+])
+def test_element_dict_round_trip(el):
+    """to_dict gives constructor arguments that rebuild the same record,
+    the fields behind properties (angle, k0, model, ...) included."""
+    d = el.to_dict()
+    cls = d.pop('__class__')
+    el2 = getattr(xt, cls)(**d)
+    assert np.array_equal(np.asarray(el.raw()[1]), np.asarray(el2.raw()[1]))

@@ -14,5 +14,6 @@ from .elements import (Marker, Drift, Multipole, Quadrupole, Bend, Sextupole,
 from .monitor import ParticlesMonitor
 from .line import Line
 from .tracker import Tracker, TrackFlags, HIPBackend
+from .slicing import Strategy, Teapot, Uniform
 
 __version__ = '0.1.0'

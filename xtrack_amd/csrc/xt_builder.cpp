@@ -124,6 +124,13 @@ void polar_consts(double h, double s, double *ca, double *sa, double *t2, double
     *ica = in_range ? 1.0 / c : 0.0;
 }
 
+// a slice of a magnet (track_magnet.h:286-560 as the slice elements call
+// it): kind -1 is the whole element, else XTM_SLICE_*
+struct SliceCfg {
+    double weight = 1.0;
+    int kind = -1;
+};
+
 struct Builder {
     std::vector<double> &prog;
     std::string &err;
@@ -307,7 +314,7 @@ struct Builder {
         return true;
     }
 
-    bool magnet(int type, const double *p) {
+    bool magnet(int type, const double *p, const SliceCfg &sl = SliceCfg{}) {
         MagnetArgs m{};
         m.order = (int64_t)p[XTM_M_ORDER];
         const int64_t nrel = (int64_t)p[XTM_M_NREL];
@@ -381,10 +388,24 @@ struct Builder {
             err = "not a magnet";
             return false;
         }
-        // forward tracking or backtracking (track_magnet.h:413-443), weight = 1:
+        // slices (elements_src/thin_slice_*.h, thick_slice_*.h, the edge
+        // slices): thin = kick only, edges off; thick = body only; entry /
+        // exit = that edge only (edge slices pass weight 0, unused)
+        bool body_active = true;
+        if (sl.kind == XTM_SLICE_THIN) {
+            m.model = -1;
+            m.edge_entry_active = 0; m.edge_exit_active = 0;
+        } else if (sl.kind == XTM_SLICE_THICK) {
+            m.edge_entry_active = 0; m.edge_exit_active = 0;
+        } else if (sl.kind == XTM_SLICE_ENTRY) {
+            body_active = false; m.edge_exit_active = 0;
+        } else if (sl.kind == XTM_SLICE_EXIT) {
+            body_active = false; m.edge_entry_active = 0;
+        }
+        // forward tracking or backtracking (track_magnet.h:413-443):
         // backtracking negates the body length, the body factor and hxl,
         // swaps the entry and exit edges and flips the linear edge kicks
-        const double weight = 1.0;
+        const double weight = sl.weight;
         const double factor_knl_ksl = 1.0;
         const double core_length = bt ? -m.length * weight : m.length * weight;
         const double factor_body = bt ? -factor_knl_ksl * weight : factor_knl_ksl * weight;
@@ -405,10 +426,19 @@ struct Builder {
                   m.edge_entry_fint, m.edge_entry_hgap, factor_edge, fbe))
             return false;
 
+        if (!body_active) {
+            if (m.edge_exit_active &&
+                !edge(m, true, m.edge_exit_model, m.edge_exit_angle, m.edge_exit_fdown,
+                      m.edge_exit_fint, m.edge_exit_hgap, factor_edge, fbe))
+                return false;
+            return true;
+        }
         int64_t integrator = m.integrator, model = m.model, nmk = m.num_multipole_kicks;
         if (integrator == 0) integrator = m.default_integrator;
         if (model == 0) model = m.default_model;
         if (model == -1) { integrator = 3; nmk = 1; }
+        // the number of kicks follows the weight (track_magnet.h:529-532)
+        if (weight != 1.0 && nmk > 0) nmk = (int64_t)std::ceil(nmk * weight);
         if (nmk == 0) {
             bool inactive = (m.h == 0 && m.k0 == 0 && m.k1 == 0 && m.k2 == 0 && m.k3 == 0 &&
                              m.k0s == 0 && m.k1s == 0 && m.k2s == 0 && m.k3s == 0);
@@ -655,6 +685,40 @@ struct Builder {
         return true;
     }
 
+    // slice elements (elements_src/{thin,thick,drift}_slice_*.h and the
+    // edge slices); the global aperture follows thick and drift slices,
+    // whose classes declare isthick = True
+    bool slice(const double *p) {
+        const double weight = p[XTM_SL_WEIGHT];
+        const int kind = (int)p[XTM_SL_KIND];
+        const int parent = (int)p[XTM_SL_PARENT_TYPE];
+        const double *q = p + XTM_SL_PARENT;
+        if (kind == XTM_SLICE_DRIFT) {
+            // Drift_single_particle (expanded) over weight x parent length
+            double plen;
+            if (parent == XTM_DRIFT) plen = q[XTM_D_LENGTH];
+            else if (parent >= XTM_MULTIPOLE && parent <= XTM_OCTUPOLE) plen = q[XTM_M_LENGTH];
+            else { err = "drift slice of an unsupported parent"; return false; }
+            Prog P(prog);
+            P.op(OP_DRIFT_EXP);
+            P.arg(bt ? -weight * plen : weight * plen);
+            global_aper = true;
+            return true;
+        }
+        if (!(parent >= XTM_MULTIPOLE && parent <= XTM_OCTUPOLE) ||
+            kind < XTM_SLICE_THIN || kind > XTM_SLICE_EXIT ||
+            ((kind == XTM_SLICE_ENTRY || kind == XTM_SLICE_EXIT) && parent == XTM_MULTIPOLE)) {
+            err = "unsupported slice";
+            return false;
+        }
+        SliceCfg sl;
+        sl.weight = weight;
+        sl.kind = kind;
+        if (!magnet(parent, q, sl)) return false;
+        global_aper = kind == XTM_SLICE_THICK;
+        return true;
+    }
+
     // ---- frame transformations (track_misalignments.h) ----------------------
     // XY_SHIFT(dx, dy): add -dx, -dy; skipped only where both additions
     // are of -0.0, which leave every value unchanged
@@ -876,7 +940,7 @@ struct Builder {
         switch (inner) {
         case XTM_MULTIPOLE: case XTM_QUADRUPOLE: case XTM_BEND: case XTM_SEXTUPOLE:
         case XTM_OCTUPOLE: case XTM_CAVITY: case XTM_LIMITRECT: case XTM_LIMITELLIPSE:
-        case XTM_DIPOLEEDGE: case XTM_RFMULTIPOLE: break;
+        case XTM_DIPOLEEDGE: case XTM_RFMULTIPOLE: case XTM_SLICE: break;
         default:
             err = "frame transformations are not supported for element type " +
                   std::to_string(inner);
@@ -954,6 +1018,7 @@ struct Builder {
             break; }
         case XTM_TRANSFORMED: ok = transformed(p); break;
         case XTM_RFMULTIPOLE: ok = rf_multipole(p); break;
+        case XTM_SLICE: ok = slice(p); break;
         default:
             err = "unknown element type " + std::to_string(type);
             ok = false;
@@ -1008,6 +1073,10 @@ size_t raw_size(int type, const double *p) {
     case XTM_LIMITRECT: return XTM_LR_SIZE;
     case XTM_LIMITELLIPSE: return XTM_LE_SIZE;
     case XTM_RFMULTIPOLE: return XTM_RFM_ARRAYS + 6 * ((int64_t)p[XTM_RFM_ORDER] + 1);
+    case XTM_SLICE: {
+        const int parent = (int)p[XTM_SL_PARENT_TYPE];
+        if (parent == XTM_SLICE || parent == XTM_TRANSFORMED) return 0;
+        return XTM_SL_PARENT + raw_size(parent, p + XTM_SL_PARENT); }
     case XTM_TRANSFORMED: {
         const int inner = (int)p[XTM_TR_INNER_TYPE];
         if (inner == XTM_TRANSFORMED) return 0;
@@ -1044,6 +1113,16 @@ int xt_build_program(const int32_t *types, const int64_t *offsets, const double 
                 return XTM_ERR_ARG;
             }
         }
+        if (type == XTM_SLICE || (type == XTM_TRANSFORMED && p[XTM_TR_INNER_TYPE] == XTM_SLICE)) {
+            const double *q = type == XTM_SLICE ? p : p + XTM_TR_INNER;
+            const double pt = q[XTM_SL_PARENT_TYPE];
+            const double *r = q + XTM_SL_PARENT;
+            if (!(pt == XTM_DRIFT || (pt >= XTM_MULTIPOLE && pt <= XTM_OCTUPOLE)) ||
+                (pt >= XTM_MULTIPOLE && (r[XTM_M_ORDER] < 0 || r[XTM_M_NREL] < 1))) {
+                err = "element " + std::to_string(i) + ": invalid slice record";
+                return XTM_ERR_ARG;
+            }
+        }
         if (type == XTM_RFMULTIPOLE && !(p[XTM_RFM_ORDER] >= 0)) {
             err = "element " + std::to_string(i) + ": invalid RF multipole order";
             return XTM_ERR_ARG;
@@ -1051,7 +1130,8 @@ int xt_build_program(const int32_t *types, const int64_t *offsets, const double 
         if (type == XTM_TRANSFORMED) {
             const double it = p[XTM_TR_INNER_TYPE];
             const double *q = p + XTM_TR_INNER;
-            if (!((it >= XTM_MULTIPOLE && it <= XTM_DIPOLEEDGE) || it == XTM_RFMULTIPOLE) ||
+            if (!((it >= XTM_MULTIPOLE && it <= XTM_DIPOLEEDGE) || it == XTM_RFMULTIPOLE ||
+                  it == XTM_SLICE) ||
                 (it >= XTM_MULTIPOLE && it <= XTM_OCTUPOLE &&
                  (q[XTM_M_ORDER] < 0 || q[XTM_M_NREL] < 1)) ||
                 (it == XTM_RFMULTIPOLE && !(q[XTM_RFM_ORDER] >= 0))) {

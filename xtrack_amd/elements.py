@@ -28,7 +28,7 @@ from .constants import UNLIMITED
 # element type ids (include/xtrack_amd.h)
 T_MARKER, T_DRIFT, T_MULTIPOLE, T_QUADRUPOLE, T_BEND = 0, 1, 2, 3, 4
 T_SEXTUPOLE, T_OCTUPOLE, T_CAVITY, T_LIMITRECT, T_LIMITELLIPSE = 5, 6, 7, 8, 9
-T_DIPOLEEDGE, T_TRANSFORMED, T_RFMULTIPOLE = 10, 11, 12
+T_DIPOLEEDGE, T_TRANSFORMED, T_RFMULTIPOLE, T_SLICE = 10, 11, 12, 13
 
 DEFAULT_MULTIPOLE_ORDER = 5          # _common.py:18
 
@@ -111,11 +111,21 @@ class BeamElement:
                              for k, v in self.__dict__.items()})
         return out
 
+    # derived quantities that the constructors reject as arguments
+    _NOT_SERIALISED = ('h',)
+
     def to_dict(self):
+        """Constructor arguments that rebuild the element: public fields,
+        and the fields held behind properties (angle, k0, model,
+        integrator, edge models, isthick, ...) under the property name."""
         out = {'__class__': type(self).__name__}
         for k, v in self.__dict__.items():
             if k.startswith('_'):
-                continue
+                name = k[1:]
+                if (name in self._NOT_SERIALISED or
+                        not isinstance(getattr(type(self), name, None), property)):
+                    continue
+                k, v = name, getattr(self, name)
             out[k] = v.tolist() if isinstance(v, np.ndarray) else v
         return out
 

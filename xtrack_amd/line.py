@@ -20,6 +20,9 @@ class Line:
             raise ValueError('elements and element_names differ in length')
         self.elements = elements
         self.element_names = list(element_names)
+        # name -> element, including elements referred to but not in the
+        # sequence (the parents of slices)
+        self.element_dict = dict(zip(self.element_names, self.elements))
         self.particle_ref = particle_ref
         self.tracker = None
         self.reset_s_at_end_turn = True
@@ -104,8 +107,15 @@ class Line:
         return Particles(**kw)
 
     def to_dict(self):
-        return {'elements': {nn: ee.to_dict() for nn, ee in zip(self.element_names, self.elements)},
+        return {'elements': {nn: ee.to_dict() for nn, ee in self.element_dict.items()},
                 'element_names': list(self.element_names)}
+
+    def slice_thick_elements(self, slicing_strategies):
+        """Replace thick elements by slices in place (xtrack line.py
+        Line.slice_thick_elements, slicing.py:215-470); returns
+        {name: [slice names]}."""
+        from .slicing import slice_line
+        return slice_line(self, slicing_strategies)
 
     @classmethod
     def from_dict(cls, dct):
@@ -115,22 +125,33 @@ class Line:
         present, becomes `line.particle_ref`.  Element classes outside this
         tracker raise NotImplementedError."""
         from .elements import _METADATA_KEYS
+        from .slicing import SLICE_CLASSES
         names = dct.get('element_names') or list(dct['elements'].keys())
-        elems = []
         built = {}
-        for nn in names:
+
+        def build(nn):
             if nn in built:                    # a name may repeat in the sequence
-                elems.append(built[nn])
-                continue
+                return built[nn]
             ed = dict(dct['elements'][nn])
             cls_name = ed.pop('__class__')
+            if cls_name in SLICE_CLASSES:
+                # slices refer to their parent by name (slice_base.py:34-44)
+                parent_name = ed.pop('parent_name')
+                for kk in _METADATA_KEYS:
+                    ed.pop(kk, None)
+                built[nn] = SLICE_CLASSES[cls_name](_parent=build(parent_name),
+                                                    parent_name=parent_name, **ed)
+                return built[nn]
             if cls_name not in ELEMENT_CLASSES:
                 raise NotImplementedError(f'element class {cls_name} ({nn}) is not supported')
             for kk in _METADATA_KEYS:
                 ed.pop(kk, None)
             built[nn] = ELEMENT_CLASSES[cls_name](**ed)
-            elems.append(built[nn])
+            return built[nn]
+
+        elems = [build(nn) for nn in names]
         line = cls(elements=elems, element_names=names)
+        line.element_dict.update(built)
         if dct.get('particle_ref'):
             line.particle_ref = Particles.from_dict(dct['particle_ref'])
         return line
