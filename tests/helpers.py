@@ -42,6 +42,31 @@ def mixed_line(n_cells=3, aperture=0.02, with_cavity=True):
     return xt.Line(elements=els)
 
 
+def segment_line(n_blocks=4, aperture=0.01):
+    """Runs of cheap elements (drifts, both aperture types, thin
+    multipoles, markers, a thin sextupole) that the line code tracks as
+    segments, cut by thick quadrupoles; the kicks and the tight apertures
+    lose particles at every position inside the runs."""
+    els = []
+    for ib in range(n_blocks):
+        s = 1 if ib % 2 == 0 else -1
+        for k in range(3):
+            els += [
+                xt.Drift(length=0.8),
+                xt.LimitRect(min_x=-aperture, max_x=aperture, min_y=-aperture, max_y=aperture),
+                xt.Multipole(knl=[0.0, 0.1 * s * (-1) ** k, 0.5], ksl=[0.0, 0.01]),
+                xt.Drift(length=0.6),
+                xt.LimitEllipse(a=aperture, b=aperture * 0.8),
+                xt.Marker(),
+                xt.Multipole(knl=[0.0, 0.0, 2.0 * s]),
+                xt.Drift(length=0.4, model='exact'),
+                xt.LimitRect(min_x=-aperture * 0.9, max_x=aperture, min_y=-aperture,
+                             max_y=aperture * 0.9),
+            ]
+        els.append(xt.Quadrupole(length=1.0, k1=0.05 * s))
+    return xt.Line(elements=els)
+
+
 def random_particles(n, seed=0, scale=1e-3, p0c=7e9, spread=1.0):
     rng = np.random.default_rng(seed)
     return xt.Particles(p0c=p0c,

@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import xtrack_amd as xt
-from helpers import mixed_line, random_particles, compare_exact, guard_edge_particles
+from helpers import (mixed_line, segment_line, random_particles, compare_exact,
+                     guard_edge_particles)
 from oracle_backend import OracleBackend, GeneratedHostBackend, generated_host_line
 from xtrack_amd.elements import build_raw_table
 
@@ -65,11 +66,63 @@ def test_collider_cells():
 
 
 def test_source_shape():
-    """One function per distinct record; elements map onto them."""
+    """One function per distinct record; elements map onto them.  Runs of
+    cheap records (drifts, apertures, thin kicks) start with a key that
+    names a segment function tracking the whole run; the records inside
+    a run keep their own keys."""
     from xtrack_amd import lattices
     line = lattices.collider_ring(k2f=0.03, k2d=-0.06)
     t, o, pp = build_raw_table(line.elements)
-    _, rec, prog, src, _ = generated_host_line(t, o, pp, line.get_length())
+    _, key, prog, src, _ = generated_host_line(t, o, pp, line.get_length())
+    rec, span, seg = key & 0x7ff, (key >> 11) & 31, key >> 16
     assert len(np.unique(rec)) == 10
     assert src.count('XT_GEN_FN void rec_') == 10
     assert 'xt::op_yoshida4(p' in src and 'xt::op_limit_rect(p' in src
+    n_seg = src.count('XT_GEN_FN int32_t seg_')
+    assert n_seg > 0 and set(np.unique(seg)) == set(range(n_seg + 1))
+    starts = np.flatnonzero(seg)
+    assert np.all(span[starts] > 1) and np.all(span[seg == 0] == 0)
+    # segments do not overlap, and the same segment covers the same records
+    assert np.all(np.diff(starts) >= span[starts][:-1])
+    names = [type(ee).__name__ for ee in line.elements]
+    for s in range(1, n_seg + 1):
+        runs = {tuple(rec[e:e + span[e]]) for e in starts[seg[starts] == s]}
+        assert len(runs) == 1
+        for e in starts[seg[starts] == s]:
+            assert not {'Bend', 'Quadrupole', 'Cavity'} & set(names[e:e + span[e]])
+    # most elements of the ring are inside segments
+    assert span.sum() > 0.8 * len(key)
+
+
+def test_segments_losses_partial_ranges_and_monitors():
+    """Runs of cheap elements tracked as segments: particles lost at every
+    aperture inside the runs (at_element must name the element inside the
+    segment), tracking ranges that start and stop inside segments, and
+    element-by-element records, which track element by element."""
+    line = segment_line()
+    t, o, pp = build_raw_table(line.elements)
+    _, key, _, _, _ = generated_host_line(t, o, pp, line.get_length())
+    span, seg = (key >> 11) & 31, key >> 16
+    pos = np.full(len(key), -1)
+    for e in np.flatnonzero(seg):
+        pos[e:e + span[e]] = np.arange(span[e])
+    assert (span > 1).sum() >= 8
+    p = random_particles(800, seed=1, scale=2e-3)
+    for kw in (dict(num_turns=3), dict(num_turns=2, ele_start=5, ele_stop=50),
+               dict(ele_start=20, num_elements=70), dict(ele_start=16, ele_stop=27),
+               dict(num_turns=2, turn_by_turn_monitor=True)):
+        lg, lc, pg, pc = _run(segment_line, p, **kw)
+        assert not compare_exact(pg, pc), kw
+        if kw.get('turn_by_turn_monitor'):
+            for ff in ('x', 'px', 'y', 'at_element', 'state'):
+                assert np.array_equal(getattr(lg.record_last_track, ff),
+                                      getattr(lc.record_last_track, ff), equal_nan=True)
+        if kw == dict(num_turns=3):
+            lost = pc.sorted_by_id()
+            at = lost['at_element'][lost['state'] <= 0]
+            assert len(np.unique(pos[at][pos[at] > 0])) >= 5
+    lg, lc, pg, pc = _run(segment_line, p, num_turns=1, turn_by_turn_monitor='ONE_TURN_EBE')
+    assert not compare_exact(pg, pc)
+    for ff in ('x', 'px', 'y', 'py', 'zeta', 'at_element', 'state'):
+        assert np.array_equal(getattr(lg.record_last_track, ff),
+                              getattr(lc.record_last_track, ff), equal_nan=True), ff

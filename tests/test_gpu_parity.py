@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import xtrack_amd as xt
-from helpers import mixed_line, random_particles, compare_exact, guard_edge_particles
+from helpers import (mixed_line, segment_line, random_particles, compare_exact,
+                     guard_edge_particles)
 from oracle_backend import OracleBackend
 
 pytestmark = pytest.mark.gpu
@@ -314,3 +315,24 @@ def test_sliced_line_bit_exact():
         lc.track(pc, num_turns=3, backtrack=bt)
         bad = compare_exact(pg, pc)
         assert not bad, (bt, bad)
+
+
+def test_segment_runs_bit_exact():
+    """Runs of cheap elements that the compiled line code tracks as
+    segments: losses at every aperture inside the runs, tracking ranges
+    that start or stop inside a segment, and element-by-element records."""
+    print('[segment runs]', flush=True)
+    p = random_particles(3000, seed=1, scale=2e-3)
+    for kw in (dict(num_turns=3), dict(num_turns=2, ele_start=5, ele_stop=50),
+               dict(ele_start=20, num_elements=70), dict(ele_start=16, ele_stop=27),
+               dict(num_turns=1, turn_by_turn_monitor='ONE_TURN_EBE')):
+        print(kw, flush=True)
+        lg, lc, pg, pc = _pair(segment_line, p)
+        lg.track(pg, **kw)
+        lc.track(pc, **kw)
+        bad = compare_exact(pg, pc)
+        assert not bad, (kw, bad)
+        if 'turn_by_turn_monitor' in kw:
+            for ff in ('x', 'px', 'y', 'py', 'zeta', 'at_element', 'state'):
+                assert np.array_equal(getattr(lg.record_last_track, ff),
+                                      getattr(lc.record_last_track, ff), equal_nan=True), ff

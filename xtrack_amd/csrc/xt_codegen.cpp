@@ -15,6 +15,7 @@
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <string>
@@ -41,12 +42,23 @@ std::string lit(double v) {
 
 const int kNargs[OP_NUM] = XT_OP_NARGS_INIT;
 
+// Records whose arithmetic is at most this many ops (a kick pass counts
+// its terms) and that hold no integrator, map or RF op are cheap enough
+// for the per-element dispatch to matter; runs of them form segments.
+constexpr int kSmallCost = 12;
+constexpr size_t kSegMax = 16;           // elements per segment (< 32)
+constexpr int64_t kSegMaxCount = 256;    // distinct segments per line
+constexpr int64_t kSegMaxBody = 2048;    // sum of their lengths
+static_assert(kSegMax <= XT_KEY_SPAN_MASK, "segment length must fit its key field");
+
 struct Gen {
     const std::vector<double> &prog;
     const int64_t base;   // program offset of the record
     const bool loads;     // constants as loads rp[k] instead of literals
     std::string out;
     std::string err;
+    int cost = 0;         // rough size of the record's arithmetic (ops, kick terms)
+    bool heavy = false;   // holds an integrator, a map or an RF op
 
     Gen(const std::vector<double> &p, int64_t b, bool l) : prog(p), base(b), loads(l) {}
 
@@ -82,6 +94,7 @@ struct Gen {
         }
         line("        xt::kick_pass_apply(p, " + w + ", dpx, dpy);");
         line("    }");
+        cost += (int)top + 1;
         return true;
     }
 
@@ -118,6 +131,11 @@ struct Gen {
             if (!in_range(pc + 1, n)) return false;
             const int64_t a = pc + 1;
             const std::string at = "prog + " + std::to_string(pc + 1);
+            cost += 1;
+            if (op == OP_POLAR || op == OP_DIPQUAD || op == OP_YOSHIDA4_BEND ||
+                op == OP_YOSHIDA6_BEND || op == OP_RF_KICK || op == OP_X_ROT || op == OP_Y_ROT ||
+                op == OP_RF_MULT)
+                heavy = true;
             switch (op) {
             case OP_DRIFT_EXP: line("xt::drift_expanded(p" + cs(a, 1) + ");"); break;
             case OP_DRIFT_EXACT: line("xt::drift_exact(p" + cs(a, 1) + ");"); break;
@@ -187,12 +205,14 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
 #else
     const bool loads = false;
 #endif
+    std::vector<char> small(k, 0);
     for (auto &kv : index) {
         Gen g(prog, kv.first, loads);
         if (!g.record(kv.first)) {
             err = "cannot generate code for the program at offset " + std::to_string(kv.first);
             return XTM_ERR_ARG;
         }
+        small[kv.second] = !g.heavy && g.cost <= kSmallCost;
         s += "XT_GEN_FN void rec_" + std::to_string(kv.second) +
              "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
              "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n"
@@ -200,12 +220,80 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
         s += g.out;
         s += "}\n\n";
     }
+
+    // Segments: runs of consecutive cheap records (drifts, apertures, thin
+    // kicks, markers) cut into chunks of at most kSegMax elements; each
+    // chunk sequence that occurs at least twice in the line becomes one
+    // function that tracks the chunk's elements in order and returns the
+    // position of the element where the particle was lost.  The key of a
+    // chunk's first element carries the segment number and the chunk
+    // length; the loop runs the chunk with one dispatch instead of one per
+    // element.  Same element calls, same order: the arithmetic is that of
+    // the single-record path.
+    std::vector<std::vector<int32_t>> segs;
+    if (!loads) {
+        std::vector<std::pair<size_t, size_t>> chunks;   // (first element, length)
+        size_t e = 0;
+        while (e < rec_of_elem.size()) {
+            if (!small[rec_of_elem[e]]) { e++; continue; }
+            size_t f = e;
+            while (f < rec_of_elem.size() && small[rec_of_elem[f]]) f++;
+            for (size_t a = e; a < f; a += kSegMax) {
+                const size_t n = std::min<size_t>(kSegMax, f - a);
+                if (n > 1) chunks.emplace_back(a, n);
+            }
+            e = f;
+        }
+        std::map<std::vector<int32_t>, int64_t> count;
+        for (auto &ch : chunks)
+            count[std::vector<int32_t>(rec_of_elem.begin() + ch.first,
+                                       rec_of_elem.begin() + ch.first + ch.second)]++;
+        std::vector<std::pair<int64_t, const std::vector<int32_t> *>> ranked;
+        for (auto &kv : count)
+            if (kv.second >= 2)
+                ranked.emplace_back(kv.second * (int64_t)(kv.first.size() - 1), &kv.first);
+        std::stable_sort(ranked.begin(), ranked.end(),
+                         [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::map<std::vector<int32_t>, int32_t> seg_id;
+        int64_t body = 0;
+        for (auto &r : ranked) {
+            if ((int64_t)segs.size() >= kSegMaxCount ||
+                body + (int64_t)r.second->size() > kSegMaxBody)
+                continue;
+            seg_id.emplace(*r.second, (int32_t)segs.size());
+            segs.push_back(*r.second);
+            body += (int64_t)r.second->size();
+        }
+        for (auto &ch : chunks) {
+            auto it = seg_id.find(std::vector<int32_t>(rec_of_elem.begin() + ch.first,
+                                                       rec_of_elem.begin() + ch.first + ch.second));
+            if (it == seg_id.end()) continue;
+            rec_of_elem[ch.first] |= ((it->second + 1) << XT_KEY_SEG_SHIFT) |
+                                     ((int32_t)ch.second << XT_KEY_SPAN_SHIFT);
+        }
+        for (size_t q = 0; q < segs.size(); q++) {
+            s += "XT_GEN_FN int32_t seg_" + std::to_string(q) +
+                 "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
+                 "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n";
+            const auto &v = segs[q];
+            for (size_t j = 0; j < v.size(); j++) {
+                s += "    rec_" + std::to_string(v[j]) + "(p, m, prog, rp, c);";
+                s += j + 1 < v.size() ? " if (p.state <= 0) return " + std::to_string(j) + ";\n"
+                                      : " return " + std::to_string(j) + ";\n";
+            }
+            s += "}\n\n";
+        }
+    }
     s += "struct Runner {\n"
          "    const int32_t *__restrict__ rec;\n"
          "    const int64_t *__restrict__ entry;\n"
          "    const double *__restrict__ prog;\n";
     if (loads)
         s += "    struct Key { int32_t r; int64_t pc; };\n"
+             "    static constexpr bool kSegments = false;\n"
+             "    XT_HD static int32_t span(Key) { return 0; }\n"
+             "    XT_HD int32_t segment(xt::PState<double> &, const xt::PRef &, Key,\n"
+             "                          const xt::TrackCtx &) const { return 0; }\n"
              "    XT_HD Key key(int64_t e) const { return Key{rec[e], entry[e]}; }\n"
              "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
              "                          const xt::TrackCtx &c) const {\n"
@@ -213,11 +301,28 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
              "        switch (k.r) {\n";
     else
         s += "    using Key = int32_t;\n"
+             "    static constexpr bool kSegments = true;\n"
              "    XT_HD Key key(int64_t e) const { return rec[e]; }\n"
-             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key r,\n"
+             "    XT_HD static int32_t span(Key k) { return (k >> " +
+             std::to_string(XT_KEY_SPAN_SHIFT) + ") & " + std::to_string(XT_KEY_SPAN_MASK) +
+             "; }\n"
+             "    XT_HD int32_t segment(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
              "                          const xt::TrackCtx &c) const {\n"
              "        const double *__restrict__ rp = prog;\n"
-             "        switch (r) {\n";
+             "        switch (k >> " + std::to_string(XT_KEY_SEG_SHIFT) + ") {\n" + [&] {
+                 std::string t;
+                 for (size_t q = 0; q < segs.size(); q++)
+                     t += "        case " + std::to_string(q + 1) + ": return seg_" +
+                          std::to_string(q) + "(p, m, prog, rp, c);\n";
+                 return t;
+             }() +
+             "        default: p.state = -999; return 0;\n"
+             "        }\n"
+             "    }\n"
+             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
+             "                          const xt::TrackCtx &c) const {\n"
+             "        const double *__restrict__ rp = prog;\n"
+             "        switch (k & " + std::to_string(XT_KEY_REC_MASK) + ") {\n";
     for (int32_t r = 0; r < k; r++)
         s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
              "(p, m, prog, rp, c); break;\n";

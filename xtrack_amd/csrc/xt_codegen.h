@@ -9,13 +9,23 @@
 // generated; lines with more run on the program interpreter.
 constexpr int64_t XT_CODEGEN_MAX_RECORDS = 2048;
 
+// Element keys of generated line code (32 bits): record index in bits
+// 0-10, segment length in bits 11-15 (0: not the start of a segment) and
+// segment number + 1 in bits 16-30 (0: none).
+constexpr int32_t XT_KEY_REC_MASK = 0x7ff;
+constexpr int32_t XT_KEY_SPAN_SHIFT = 11;
+constexpr int32_t XT_KEY_SPAN_MASK = 31;
+constexpr int32_t XT_KEY_SEG_SHIFT = 16;
+static_assert(XT_CODEGEN_MAX_RECORDS <= XT_KEY_REC_MASK + 1, "record index must fit its key field");
+
 // Source of a translation unit that tracks the line: one function per
 // distinct element record with the record's constants as literals, an
 // element runner switching on the record index of each element, and (when
 // compiled as HIP) the kernel `xtm_line_kernel`, whose parameters are those
 // of the interpreter kernel with the entry array replaced by `rec`.  The
-// source includes xt_ops.h.  `rec_of_elem[e]` receives the record index of
-// element e.  `waves_per_eu` (> 0) sets the kernel's occupancy target.
+// source includes xt_ops.h.  `rec_of_elem[e]` receives the key of element
+// e (its record index and, at the start of a segment of cheap records, the
+// segment fields above).  `waves_per_eu` (> 0) sets the kernel's occupancy target.
 // Returns an XTM_* status and sets `err` on failure.
 int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
                             int32_t waves_per_eu, std::string &src,

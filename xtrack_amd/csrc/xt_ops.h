@@ -1164,11 +1164,26 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
                current element's arithmetic */
             typename Runner::Key key{};
             if (ele_start < ele_stop) key = run.key(ele_start);
-            for (int64_t e = ele_start; e < ele_stop; e++) {
-                const typename Runner::Key next = run.key(e + 1 < ele_stop ? e + 1 : e);
+            for (int64_t e = ele_start; e < ele_stop;) {
+                /* a runner with segments tracks a run of n consecutive
+                   elements in one call (not with element-by-element
+                   records) and returns the position j in the run of the
+                   element where the particle was lost */
+                int64_t n = 1, j = 0;
+                if constexpr (Runner::kSegments) {
+                    const int32_t span = Runner::span(key);
+                    if (span > 1 && flag_monitor != 2 && e + span <= ele_stop) n = span;
+                }
+                const typename Runner::Key next = run.key(e + n < ele_stop ? e + n : e);
                 if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
-                run(p, m, key, c);
-                if (p.state <= 0) { e_lost = e; break; }
+                if constexpr (Runner::kSegments) {
+                    if (n > 1) j = run.segment(p, m, key, c);
+                    else run(p, m, key, c);
+                } else {
+                    run(p, m, key, c);
+                }
+                if (p.state <= 0) { e_lost = e + j; break; }
+                e += n;
                 key = next;
             }
             if (e_lost >= 0) break;
@@ -1220,6 +1235,7 @@ struct ProgramRunner {
     const int64_t *__restrict__ entry;
     const double *__restrict__ prog;
     using Key = int64_t;
+    static constexpr bool kSegments = false;
     XT_HD Key key(int64_t e) const { return entry[e]; }
     XT_HD void operator()(PState<double> &p, const PRef &m, Key pc, const TrackCtx &c) const {
         run_element(p, m, prog, pc, c);
