@@ -79,9 +79,11 @@ def parse():
 def _read_counter(outdir, counter):
     """(copy_value, track_value) of `counter` from a rocprofv3 csv run: the
     copy is the calibration dispatch (kernel name holds 'neg'), the tracking
-    value is that of the last tracking dispatch (the timed launch) of either
-    kernel: compiled line code (xtm_line_kernel) or the interpreter
-    (track_kernel)."""
+    value is that of the last dispatch (the timed launch) of the kernel that
+    tracked the beam: the compiled line code (xtm_line_kernel) when the line
+    is compiled, else the interpreter (track_kernel).  With compiled code the
+    interpreter kernel is still launched after each compiled launch, for the
+    particles with chi != 1 (none here), and is not the one measured."""
     files = glob.glob(os.path.join(outdir, '**', '*counter_collection.csv'), recursive=True)
     if not files:
         raise RuntimeError(f'no counter_collection.csv under {outdir}')
@@ -98,7 +100,9 @@ def _read_counter(outdir, counter):
                 per_dispatch[key] = (name, old[1] + val)
     items = sorted(per_dispatch.items(), key=lambda kv: kv[0][1])
     copy = [v for _, (nm, v) in items if 'neg' in nm]
-    track = [v for _, (nm, v) in items if 'xtm_line_kernel' in nm or 'track_kernel' in nm]
+    track = [v for _, (nm, v) in items if 'xtm_line_kernel' in nm]
+    if not track:
+        track = [v for _, (nm, v) in items if 'track_kernel' in nm]
     if not copy or not track:
         raise RuntimeError(f'{counter}: calibration or tracking dispatch not found')
     return max(copy), track[-1]
@@ -273,7 +277,8 @@ def main():
                      'flop_per_particle_turn': flop_pt,
                      'ops_per_particle_turn': {'add': ops[0], 'mul': ops[1], 'div': ops[2],
                                                'sqrt': ops[3], 'calls': ops[4]},
-                     'kernel_ms_mean': float(np.mean(kern_ms))},
+                     'kernel_ms_mean': float(np.mean(kern_ms)),
+                     'kernel_ms': [round(float(x), 3) for x in kern_ms]},
     }
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(make_line, tw, args.cpu_sample_particles,

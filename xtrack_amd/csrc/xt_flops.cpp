@@ -82,12 +82,15 @@ int xt_count_turn(const std::vector<int64_t> &entry, const std::vector<double> &
     const double beta0 = p0c / energy0;
     const double delta_beta0 = delta * beta0;
     const double ptau_beta0 = std::sqrt(delta_beta0 * delta_beta0 + 2 * delta_beta0 * beta0 + 1) - 1;
-    p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.delta = delta;
-    p.rpp = 1 / (1 + delta);
-    p.rvv = (1 + delta) / (1 + ptau_beta0);
+    FC slow[xt::XS_N];
+    p.slow = slow;
+    p.slow_stride = 1;
+    p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.put(xt::XS_DELTA, FC(delta));
+    p.put(xt::XS_RPP, FC(1 / (1 + delta)));
+    p.put(xt::XS_RVV, FC((1 + delta) / (1 + ptau_beta0)));
     p.s = 0.0;
     p.chi = 1.0;
-    p.inv_rvv = FC(1.0) / p.rvv;
+    p.put(xt::XS_INV_RVV, FC(1.0) / p.rvv());
     p.at_turn = 0; p.state = 1;
     xt::refresh_flags(p);
     // fields the maps read from and write to the particle arrays

@@ -65,13 +65,24 @@ enum { KF_P1 = 1, KF_P2 = 2, KF_P3 = 4, KF_HL = 8, KF_G0 = 16, KF_H1 = 32 };
    and are read and written there (PRef); at_element is not held at all:
    every element passed adds one, so it is a per-particle base plus the
    element index (track_particle). */
+/* slots of the energy fields in PState::slow */
+enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_N = 4 };
+
 template <class R>
 struct PState {
-    R x, px, y, py, zeta, delta, rpp, rvv, s;
+    R x, px, y, py, zeta, s;
     R chi;
-    /* 1./rvv, kept equal to that IEEE quotient: set when the particle is
-       loaded and whenever rvv changes (update_ptau) */
-    R inv_rvv;
+    /* delta, rpp, rvv and 1./rvv change only when the energy changes (RF
+       kicks) but are read by most elements.  They live in per-thread
+       memory, slot k at slow[k * slow_stride] (LDS in the kernels), and
+       are read there at each use: held in registers they would be live
+       across the whole element loop, and with the loop's out-of-line calls
+       the register allocator keeps them in scratch memory instead and
+       writes them back on every element.  inv_rvv is kept equal to the
+       IEEE quotient 1./rvv: set when the particle is loaded and whenever
+       rvv changes (update_ptau). */
+    R *slow;
+    int32_t slow_stride;
     int64_t at_turn, state;
     /* per-particle predicates of quantities that change only at load and in
        update_ptau (refresh_flags): the folded multipole kick may be used
@@ -79,6 +90,15 @@ struct PState {
        are in the ranges of the polar drift's reciprocal-division form
        (delta in [-0.5, 1], rvv in [0.5, 2]) */
     bool fast_ok, polar_ok;
+
+    XT_HD R get(int k) const {
+        return slow[k * slow_stride];
+    }
+    XT_HD void put(int k, const R &v) const { slow[k * slow_stride] = v; }
+    XT_HD R delta() const { return get(XS_DELTA); }
+    XT_HD R rpp() const { return get(XS_RPP); }
+    XT_HD R rvv() const { return get(XS_RVV); }
+    XT_HD R inv_rvv() const { return get(XS_INV_RVV); }
 };
 
 /* the particle's slot in the arrays, for the fields not held in PState */
@@ -161,10 +181,10 @@ __device__ __forceinline__ double sqrt_ir(double x) {
 
 template <class R>
 XT_HD void refresh_flags(PState<R> &p) {
-    p.fast_ok = (xval(p.chi) == 1.0) && xfinite(xval(p.delta)) && xfinite(xval(p.rvv)) &&
-                (xval(p.rvv) != 0.0) && xfinite(xval(p.inv_rvv));
-    p.polar_ok = xval(p.delta) >= -0.5 && xval(p.delta) <= 1.0 && xval(p.rvv) >= 0.5 &&
-                 xval(p.rvv) <= 2.0;
+    p.fast_ok = (xval(p.chi) == 1.0) && xfinite(xval(p.delta())) && xfinite(xval(p.rvv())) &&
+                (xval(p.rvv()) != 0.0) && xfinite(xval(p.inv_rvv()));
+    p.polar_ok = xval(p.delta()) >= -0.5 && xval(p.delta()) <= 1.0 && xval(p.rvv()) >= 0.5 &&
+                 xval(p.rvv()) <= 2.0;
 }
 
 /* Lane condition of the polar drift's reciprocal-division form: with the
@@ -188,8 +208,8 @@ XT_HD bool polar_fast_ok(const PState<R> &p, const double ica) {
 /* track_drift.h:11-23 */
 template <class R>
 XT_HD void drift_expanded(PState<R> &p, const double length) {
-    R const rpp = p.rpp;
-    R const rv0v = p.inv_rvv;
+    R const rpp = p.rpp();
+    R const rv0v = p.inv_rvv();
     R const xp = p.px * rpp;
     R const yp = p.py * rpp;
     R const dzeta = 1 - rv0v * (1. + (xp * xp + yp * yp) / 2.);
@@ -203,8 +223,8 @@ XT_HD void drift_expanded(PState<R> &p, const double length) {
 template <class R>
 XT_HD void drift_exact(PState<R> &p, const double length) {
     R const px = p.px, py = p.py;
-    R const rv0v = p.inv_rvv;
-    R const one_plus_delta = 1. + p.delta;
+    R const rv0v = p.inv_rvv();
+    R const one_plus_delta = 1. + p.delta();
     R const one_over_pz = 1. / xsqrt(one_plus_delta * one_plus_delta - px * px - py * py);
     R const dzeta = 1 - rv0v * one_plus_delta * one_over_pz;
     p.x = p.x + px * one_over_pz * length;
@@ -294,9 +314,9 @@ __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double l
                                                const double rho, const double ca,
                                                const double sa, const double t2,
                                                const double ica) {
-    const double rvv = p.rvv;
+    const double rvv = p.rvv();
     const double x = p.x, y = p.y, px = p.px, py = p.py;
-    const double one_plus_delta = p.delta + 1.0;
+    const double one_plus_delta = p.delta() + 1.0;
     const double rad = one_plus_delta * one_plus_delta - px * px - py * py;
     const double pz = sqrt_ir(rad);
     const double _pz = rcp_ir(pz);
@@ -315,7 +335,7 @@ __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double l
     const double q4 = div_rcp(div_rcp(n3, ca, ica), pz, _pz);
     const double den = 1 - q4;
     const double delta_ell = div_ir(q2, den);
-    double dz = length - div_rcp(delta_ell, rvv, p.inv_rvv);
+    double dz = length - div_rcp(delta_ell, rvv, p.inv_rvv());
     if (!polar_fast_ok(p, ica)) {
         const PolarOut o = polar_plain(x, px, y, py, one_plus_delta, rvv, length, rho, ca, sa, t2);
         new_x = o.x;
@@ -343,9 +363,9 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
         return;
     }
 #endif
-    const R rvv = p.rvv;
+    const R rvv = p.rvv();
     const R x = p.x, y = p.y, px = p.px, py = p.py;
-    const R one_plus_delta = p.delta + 1.0;
+    const R one_plus_delta = p.delta() + 1.0;
     const R pz = xsqrt(one_plus_delta * one_plus_delta - px * px - py * py);
     const R _pz = 1 / pz;
     const R pxt = px * _pz;
@@ -359,7 +379,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
     const R n1 = one_plus_delta * xr * sa;
     const R n3 = px * sa;
-    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv, polar_fast_ok(p, ica), length);
+    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv(), polar_fast_ok(p, ica), length);
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;
@@ -452,7 +472,7 @@ XT_HD_COLD DQOut<R> dipole_quad_map(const R x, const R px, const R y, const R py
 template <class R>
 XT_HD void dipole_quad(PState<R> &p, const double length, const double k0, const double k1,
                        const double h) {
-    const DQOut<R> o = dipole_quad_map(p.x, p.px, p.y, p.py, p.delta, p.rvv, p.chi, length,
+    const DQOut<R> o = dipole_quad_map(p.x, p.px, p.y, p.py, p.delta(), p.rvv(), p.chi, length,
                                        k0, k1, h);
     p.x = o.x;
     p.px = o.px;
@@ -544,8 +564,8 @@ XT_HD bool kick_sane(const PState<R> &p) {
 /* the literal kick for lanes that fail kick_sane; d = kick descriptor */
 template <class R>
 XT_HD void kick_literal_apply(PState<R> &p, double w, const double *__restrict__ d) {
-    const KickOut<R> o = magnet_kick_literal(p.x, p.y, p.px, p.py, p.zeta, p.delta, p.chi,
-                                             p.inv_rvv, w, d);
+    const KickOut<R> o = magnet_kick_literal(p.x, p.y, p.px, p.py, p.zeta, p.delta(), p.chi,
+                                             p.inv_rvv(), w, d);
     p.px = o.px;
     p.py = o.py;
     p.zeta = o.zeta;
@@ -577,9 +597,9 @@ XT_HD void kick_curvature(PState<R> &p, const R x, const R y, const bool has_hl,
     R dpx = R(0.0), dpy = R(0.0), dzeta = R(0.0);
     bool hx = false;
     if (has_hl) {
-        dpx = hl * (1. + p.delta);
+        dpx = hl * (1. + p.delta());
         hx = true;
-        R const rv0v = p.inv_rvv;
+        R const rv0v = p.inv_rvv();
         dzeta = -rv0v * hl * x;
     }
     if (has_g0) {
@@ -641,11 +661,11 @@ XT_HD void update_ptau(PState<R> &p, R new_ptau, const PRef &m) {
     R const irpp = xsqrt(ptau * ptau + 2 * ptau / beta0 + 1);
     R const new_rpp = 1. / irpp;
     R const new_rvv = irpp / (1 + beta0 * ptau);
-    p.delta = irpp - 1;
-    p.rvv = new_rvv;
+    p.put(XS_DELTA, irpp - 1);
+    p.put(XS_RVV, new_rvv);
     m.P->ptau[m.i] = xval(ptau);
-    p.rpp = new_rpp;
-    p.inv_rvv = 1. / new_rvv;
+    p.put(XS_RPP, new_rpp);
+    p.put(XS_INV_RVV, 1. / new_rvv);
     refresh_flags(p);
 }
 
@@ -1052,9 +1072,9 @@ XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i
     D.px[store_at] = p.px;
     D.py[store_at] = p.py;
     D.ptau[store_at] = P.ptau[i];
-    D.delta[store_at] = p.delta;
-    D.rpp[store_at] = p.rpp;
-    D.rvv[store_at] = p.rvv;
+    D.delta[store_at] = p.delta();
+    D.rpp[store_at] = p.rpp();
+    D.rvv[store_at] = p.rvv();
     D.chi[store_at] = p.chi;
     D.charge_ratio[store_at] = P.charge_ratio[i];
     D.weight[store_at] = P.weight[i];
@@ -1087,8 +1107,12 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
        backtracking (element-by-element records are) */
     if (backtrack && !M.ebe_mode) return;
     PState<double> p;
-    p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.delta = delta;
-    p.rpp = rpp; p.rvv = rvv; p.s = s; p.chi = chi; p.at_turn = at_turn; p.state = state;
+    double slow[XS_N];
+    p.slow = slow;
+    p.slow_stride = 1;
+    p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.put(XS_DELTA, delta);
+    p.put(XS_RPP, rpp); p.put(XS_RVV, rvv); p.s = s; p.chi = chi; p.at_turn = at_turn;
+    p.state = state;
     const int64_t pid = P.particle_id[i];
     const int64_t n_turns_record = M.stop_at_turn - M.start_at_turn;
     const int64_t at_turn_m = M.ebe_mode ? at_element : at_turn;
@@ -1122,8 +1146,8 @@ struct MonRef {
 };
 
 #define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
-    monitor_record(*mon.M, *mon.P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta, p.rpp,       \
-                   p.rvv, p.s, p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
+    monitor_record(*mon.M, *mon.P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta(), p.rpp(),       \
+                   p.rvv(), p.s, p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
 
 /* ---- one particle through [ele_start, ele_start+num_ele_track) for
    num_turns turns: the per-thread body of track_line (tracker.py:596-734)
@@ -1144,12 +1168,26 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
                                int32_t num_ele_track, int32_t flag_end_turn_actions,
                                int32_t flag_reset_s, int32_t flag_monitor, const MonRef &mon) {
     PState<double> p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* the kernels run 256 threads per block */
+    __shared__ double slow_lds[XS_N * 256];
+    p.slow = slow_lds + threadIdx.x;
+    p.slow_stride = 256;
+#else
+    double slow_buf[XS_N];
+    p.slow = slow_buf;
+    p.slow_stride = 1;
+#endif
     p.x = P.x[i]; p.px = P.px[i]; p.y = P.y[i]; p.py = P.py[i];
-    p.zeta = P.zeta[i]; p.delta = P.delta[i];
-    p.rpp = P.rpp[i]; p.rvv = P.rvv[i]; p.s = P.s[i];
+    p.zeta = P.zeta[i];
+    const double rvv0 = P.rvv[i];
+    p.put(XS_DELTA, P.delta[i]);
+    p.put(XS_RPP, P.rpp[i]);
+    p.put(XS_RVV, rvv0);
+    p.s = P.s[i];
     p.chi = CHI1 ? 1.0 : P.chi[i];
     p.at_turn = P.at_turn[i]; p.state = P.state[i];
-    p.inv_rvv = 1. / p.rvv;
+    p.put(XS_INV_RVV, 1. / rvv0);
     refresh_flags(p);
     const PRef m{&P, i};
     const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
@@ -1224,8 +1262,8 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
         at_element = e_lost >= 0 ? base + e_lost : at_el;
     }
     P.x[i] = p.x; P.px[i] = p.px; P.y[i] = p.y; P.py[i] = p.py;
-    P.zeta[i] = p.zeta; P.delta[i] = p.delta;
-    P.rpp[i] = p.rpp; P.rvv[i] = p.rvv; P.s[i] = p.s;
+    P.zeta[i] = p.zeta; P.delta[i] = p.delta();
+    P.rpp[i] = p.rpp(); P.rvv[i] = p.rvv(); P.s[i] = p.s;
     P.at_element[i] = at_element;
     P.at_turn[i] = p.at_turn; P.state[i] = p.state;
 }
