@@ -88,7 +88,7 @@ int xt_count_turn(const std::vector<int64_t> &entry, const std::vector<double> &
     p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.put(xt::XS_DELTA, FC(delta));
     p.put(xt::XS_RPP, FC(1 / (1 + delta)));
     p.put(xt::XS_RVV, FC((1 + delta) / (1 + ptau_beta0)));
-    p.s = 0.0;
+    p.set_s(FC(0.0));
     p.chi = 1.0;
     p.put(xt::XS_INV_RVV, FC(1.0) / p.rvv());
     p.at_turn = 0; p.state = 1;

@@ -66,21 +66,22 @@ enum { KF_P1 = 1, KF_P2 = 2, KF_P3 = 4, KF_HL = 8, KF_G0 = 16, KF_H1 = 32 };
    every element passed adds one, so it is a per-particle base plus the
    element index (track_particle). */
 /* slots of the energy fields in PState::slow */
-enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_N = 4 };
+enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_S = 4, XS_N = 5 };
 
 template <class R>
 struct PState {
-    R x, px, y, py, zeta, s;
+    R x, px, y, py, zeta;
     R chi;
     /* delta, rpp, rvv and 1./rvv change only when the energy changes (RF
-       kicks) but are read by most elements.  They live in per-thread
-       memory, slot k at slow[k * slow_stride] (LDS in the kernels), and
-       are read there at each use: held in registers they would be live
-       across the whole element loop, and with the loop's out-of-line calls
-       the register allocator keeps them in scratch memory instead and
-       writes them back on every element.  inv_rvv is kept equal to the
-       IEEE quotient 1./rvv: set when the particle is loaded and whenever
-       rvv changes (update_ptau). */
+       kicks) but are read by most elements; s only advances by element
+       lengths.  They live in per-thread memory, slot k at
+       slow[k * slow_stride] (LDS in the kernels), and are read there by
+       each element: held in registers they would be live across the whole
+       element loop, and with the loop's out-of-line calls the register
+       allocator keeps them in scratch memory instead and writes them back
+       on every element.  inv_rvv is kept equal to the IEEE quotient
+       1./rvv: set when the particle is loaded and whenever rvv changes
+       (update_ptau). */
     R *slow;
     int32_t slow_stride;
     int64_t at_turn, state;
@@ -99,6 +100,8 @@ struct PState {
     XT_HD R rpp() const { return get(XS_RPP); }
     XT_HD R rvv() const { return get(XS_RVV); }
     XT_HD R inv_rvv() const { return get(XS_INV_RVV); }
+    XT_HD R s() const { return get(XS_S); }
+    XT_HD void set_s(const R &v) const { put(XS_S, v); }
 };
 
 /* the particle's slot in the arrays, for the fields not held in PState */
@@ -215,7 +218,7 @@ XT_HD void drift_expanded(PState<R> &p, const double length) {
     R const dzeta = 1 - rv0v * (1. + (xp * xp + yp * yp) / 2.);
     p.x = p.x + xp * length;
     p.y = p.y + yp * length;
-    p.s = p.s + length;
+    p.set_s(p.s() + length);
     p.zeta = p.zeta + length * dzeta;
 }
 
@@ -230,7 +233,7 @@ XT_HD void drift_exact(PState<R> &p, const double length) {
     p.x = p.x + px * one_over_pz * length;
     p.y = p.y + py * one_over_pz * length;
     p.zeta = p.zeta + dzeta * length;
-    p.s = p.s + length;
+    p.set_s(p.s() + length);
 }
 
 /* Path-length term of the polar drift, the reference's literal divisions:
@@ -347,7 +350,7 @@ __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double l
     p.px = new_px;
     p.y = new_y;
     p.zeta = p.zeta + dz;
-    p.s = p.s + length;
+    p.set_s(p.s() + length);
 }
 #endif
 
@@ -384,7 +387,7 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     p.px = new_px;
     p.y = new_y;
     p.zeta = p.zeta + dz;
-    p.s = p.s + length;
+    p.set_s(p.s() + length);
 }
 
 template <class R>
@@ -479,7 +482,7 @@ XT_HD void dipole_quad(PState<R> &p, const double length, const double k0, const
     p.y = o.y;
     p.py = o.py;
     p.zeta = p.zeta + o.dzeta;
-    p.s = p.s + length;
+    p.set_s(p.s() + length);
 }
 
 /* ---- multipole kick (track_magnet_kick.h:24-144, 183-228) ----------------- */
@@ -874,7 +877,7 @@ template <class R>
 XT_HD void op_s_shift(PState<R> &p, const double ds) {
     drift_exact(p, ds);
     p.zeta = p.zeta + (-ds);
-    p.s = p.s + (-ds);
+    p.set_s(p.s() + (-ds));
 }
 
 /* track_srotation.h:11-40 (spin is not tracked) */
@@ -1065,7 +1068,7 @@ XT_HD void monitor_store(const xtm_monitor &M, const xtm_particles &P, int64_t i
     D.p0c[store_at] = P.p0c[i];
     D.gamma0[store_at] = P.gamma0[i];
     D.beta0[store_at] = P.beta0[i];
-    D.s[store_at] = p.s;
+    D.s[store_at] = p.s();
     D.zeta[store_at] = p.zeta;
     D.x[store_at] = p.x;
     D.y[store_at] = p.y;
@@ -1111,7 +1114,7 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
     p.slow = slow;
     p.slow_stride = 1;
     p.x = x; p.px = px; p.y = y; p.py = py; p.zeta = zeta; p.put(XS_DELTA, delta);
-    p.put(XS_RPP, rpp); p.put(XS_RVV, rvv); p.s = s; p.chi = chi; p.at_turn = at_turn;
+    p.put(XS_RPP, rpp); p.put(XS_RVV, rvv); p.set_s(s); p.chi = chi; p.at_turn = at_turn;
     p.state = state;
     const int64_t pid = P.particle_id[i];
     const int64_t n_turns_record = M.stop_at_turn - M.start_at_turn;
@@ -1147,7 +1150,7 @@ struct MonRef {
 
 #define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
     monitor_record(*mon.M, *mon.P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta(), p.rpp(),       \
-                   p.rvv(), p.s, p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
+                   p.rvv(), p.s(), p.chi, p.at_turn, p.state, (AT_ELEMENT), BT)
 
 /* ---- one particle through [ele_start, ele_start+num_ele_track) for
    num_turns turns: the per-thread body of track_line (tracker.py:596-734)
@@ -1184,7 +1187,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
     p.put(XS_DELTA, P.delta[i]);
     p.put(XS_RPP, P.rpp[i]);
     p.put(XS_RVV, rvv0);
-    p.s = P.s[i];
+    p.set_s(P.s[i]);
     p.chi = CHI1 ? 1.0 : P.chi[i];
     p.at_turn = P.at_turn[i]; p.state = P.state[i];
     p.put(XS_INV_RVV, 1. / rvv0);
@@ -1229,7 +1232,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
             if (flag_end_turn_actions > 0) {
                 p.at_turn += 1;
                 base = -(int64_t)ele_start;   /* at_element = 0 at the next turn's start */
-                if (flag_reset_s > 0) p.s = 0.;
+                if (flag_reset_s > 0) p.set_s(0.);
             } else {
                 base += num_ele_track;
             }
@@ -1246,7 +1249,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
             if (flag_end_turn_actions > 0) {
                 p.at_turn -= 1;
                 at_el = c.num_ele_line;
-                if (flag_reset_s > 0) p.s = c.line_length;
+                if (flag_reset_s > 0) p.set_s(c.line_length);
             }
             base = at_el - (ele_stop - 1);
             for (int64_t e = ele_stop - 1; e >= ele_start; e--) {
@@ -1263,7 +1266,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
     }
     P.x[i] = p.x; P.px[i] = p.px; P.y[i] = p.y; P.py[i] = p.py;
     P.zeta[i] = p.zeta; P.delta[i] = p.delta();
-    P.rpp[i] = p.rpp(); P.rvv[i] = p.rvv(); P.s[i] = p.s;
+    P.rpp[i] = p.rpp(); P.rvv[i] = p.rvv(); P.s[i] = p.s();
     P.at_element[i] = at_element;
     P.at_turn[i] = p.at_turn; P.state[i] = p.state;
 }
