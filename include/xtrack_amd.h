@@ -313,6 +313,43 @@ int32_t xtm_track_line(xtm_line *line, const xtm_particles *particles,
 int32_t xtm_line_count_ops(const xtm_line *line, const double *coords, double p0c,
                            double mass0, double q0, double *out);
 
+/* ---- batched lattice variants ---------------------------------------------
+ * Many lines (lattice variants with their own element counts and
+ * strengths) tracked in one launch, each over its own contiguous range of
+ * one particle set.  No counterpart in the reference, which tracks one
+ * line per `Line.track` call (xtrack/tracker.py:1372, one `track_line`
+ * launch per call); each variant's particles get exactly what that call
+ * gives them for full turns from element 0 (end-of-turn actions on, s
+ * reset at the end of each turn, no monitor).  The lines run on the
+ * element-program interpreter. */
+typedef struct xtm_batch xtm_batch;
+
+/* A batch over `num_lines` line handles (all on one device, or all
+ * host-only for layout inspection).  The lines must outlive the batch. */
+int32_t xtm_batch_create(xtm_line *const *lines, int32_t num_lines, xtm_batch **out);
+int32_t xtm_batch_destroy(xtm_batch *batch);
+
+/* Particles of line k are [line_start[k], line_start[k+1]) of the particle
+ * set passed to xtm_batch_track (line_start has num_lines + 1 entries,
+ * non-decreasing, from 0).  block_size (256 threads: the per-thread LDS
+ * slots of the kernels are laid out for it) and waves_per_eu (3, 4 or 5,
+ * the occupancy the kernel instance was compiled for) select the kernel
+ * instance.  Builds the block table: each block holds particles of one
+ * line only, and blocks of longer lines come first. */
+int32_t xtm_batch_set_layout(xtm_batch *batch, const int64_t *line_start,
+                             int32_t block_size, int32_t waves_per_eu);
+
+/* Number of blocks of the current layout; the table itself as
+ * (line, first particle, count) triples (3 * num_blocks int64). */
+int64_t xtm_batch_num_blocks(const xtm_batch *batch);
+int32_t xtm_batch_copy_blocks(const xtm_batch *batch, int64_t *out);
+
+/* Track num_turns full turns, asynchronously on `stream`.  The particle set
+ * must hold at least line_start[num_lines] particles.  Launches kernels
+ * only (no allocation, no copy), so it may be captured in a HIP graph. */
+int32_t xtm_batch_track(xtm_batch *batch, const xtm_particles *particles,
+                        int32_t num_turns, uint64_t track_flags, void *stream);
+
 /* Wait for all work queued on `stream`. */
 int32_t xtm_synchronize(void *stream);
 

@@ -15,5 +15,6 @@ from .monitor import ParticlesMonitor
 from .line import Line
 from .tracker import Tracker, TrackFlags, HIPBackend
 from .slicing import Strategy, Teapot, Uniform
+from .batch import BatchTracker
 
 __version__ = '0.1.0'

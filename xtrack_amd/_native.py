@@ -136,6 +136,19 @@ def load():
     lib.xtm_line_copy_program.argtypes = [c.c_void_p, c.c_void_p]
     lib.xtm_line_copy_entry.restype = c.c_int32
     lib.xtm_line_copy_entry.argtypes = [c.c_void_p, c.c_void_p]
+    lib.xtm_batch_create.restype = c.c_int32
+    lib.xtm_batch_create.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_void_p)]
+    lib.xtm_batch_destroy.restype = c.c_int32
+    lib.xtm_batch_destroy.argtypes = [c.c_void_p]
+    lib.xtm_batch_set_layout.restype = c.c_int32
+    lib.xtm_batch_set_layout.argtypes = [c.c_void_p, c.c_void_p, c.c_int32, c.c_int32]
+    lib.xtm_batch_num_blocks.restype = c.c_int64
+    lib.xtm_batch_num_blocks.argtypes = [c.c_void_p]
+    lib.xtm_batch_copy_blocks.restype = c.c_int32
+    lib.xtm_batch_copy_blocks.argtypes = [c.c_void_p, c.c_void_p]
+    lib.xtm_batch_track.restype = c.c_int32
+    lib.xtm_batch_track.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
+                                    c.c_uint64, c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

@@ -16,6 +16,7 @@
 #include <hip/hiprtc.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -71,6 +72,35 @@ __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     xt::track_particle(P, i, entry, prog, line_length, num_ele_line, num_turns, ele_start,
                        num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor,
                        xt::MonRef{Md, Pd}, flags);
+}
+
+/* One block of a batch: particles [first, first + count) of line `line`. */
+struct BatchBlock {
+    int64_t first;
+    int32_t line;
+    int32_t count;
+};
+
+/* Batched lattice variants: block b tracks its particles through its line
+   (element programs entries[line], progs[line]) for num_turns full turns.
+   The block's line is uniform across the block, so element dispatch stays
+   a scalar branch as in track_kernel. */
+template <int BS, int WPE>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void batch_kernel(
+    const BatchBlock *__restrict__ blocks, const int64_t *const *__restrict__ entries,
+    const double *const *__restrict__ progs, const int64_t *__restrict__ nele,
+    const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,
+    int32_t num_turns, uint64_t flags) {
+    static_assert(BS <= BLOCK, "the per-thread LDS slots are laid out for 256 threads");
+    const BatchBlock b = blocks[blockIdx.x];
+    if ((int32_t)threadIdx.x >= b.count) return;
+    const xtm_particles &P = *Pd;
+    const int64_t i = b.first + threadIdx.x;
+    const int64_t ne = nele[b.line];
+    const xt::TrackCtx c{flags, lengths[b.line], P.q0, P.t_sim, ne};
+    const xt::ProgramRunner run{entries[b.line], progs[b.line]};
+    xt::track_particle_with<false>(P, i, run, run, c, num_turns, 0, (int32_t)ne, 1, 1, 0,
+                                   xt::MonRef{nullptr, Pd});
 }
 
 /* device copies of the particle struct and the monitor for the tracking
@@ -138,6 +168,24 @@ struct xtm_line {
     int32_t *d_rec = nullptr;
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;
+};
+
+struct xtm_batch {
+    int32_t device;
+    std::vector<xtm_line *> lines;
+    std::vector<int64_t> nele;
+    std::vector<int64_t> line_start;      // current layout (num_lines + 1)
+    std::vector<BatchBlock> blocks;       // current block table
+    int32_t block_size = 0, waves_per_eu = 0;
+    // device tables
+    const int64_t **d_entries = nullptr;
+    const double **d_progs = nullptr;
+    int64_t *d_nele = nullptr;
+    double *d_lengths = nullptr;
+    BatchBlock *d_blocks = nullptr;
+    int64_t d_blocks_cap = 0;
+    xtm_particles *d_p = nullptr;
+    xtm_monitor *d_m = nullptr;
 };
 
 #ifndef XTM_RTC_DEFINES
@@ -448,6 +496,177 @@ int32_t xtm_line_count_ops(const xtm_line *L, const double *coords, double p0c,
     if (!L || !coords || !out) return fail(XTM_ERR_ARG, "xtm_line_count_ops: null argument");
     int st = xt_count_turn(L->entry, L->prog, L->line_length, coords, p0c, mass0, q0, out);
     if (st != XTM_OK) return fail(st, "xtm_line_count_ops: probe particle lost");
+    return XTM_OK;
+}
+
+int32_t xtm_batch_destroy(xtm_batch *B) {
+    if (!B) return XTM_OK;
+    if (B->device >= 0) {
+        (void)hipSetDevice(B->device);
+        if (B->d_entries) (void)hipFree(B->d_entries);
+        if (B->d_progs) (void)hipFree(B->d_progs);
+        if (B->d_nele) (void)hipFree(B->d_nele);
+        if (B->d_lengths) (void)hipFree(B->d_lengths);
+        if (B->d_blocks) (void)hipFree(B->d_blocks);
+        if (B->d_p) (void)hipFree(B->d_p);
+        if (B->d_m) (void)hipFree(B->d_m);
+    }
+    delete B;
+    return XTM_OK;
+}
+
+int32_t xtm_batch_create(xtm_line *const *lines, int32_t num_lines, xtm_batch **out) {
+    if (!out || !lines || num_lines <= 0)
+        return fail(XTM_ERR_ARG, "xtm_batch_create: invalid arguments");
+    *out = nullptr;
+    const int32_t device = lines[0] ? lines[0]->device : -1;
+    for (int32_t k = 0; k < num_lines; k++) {
+        if (!lines[k]) return fail(XTM_ERR_ARG, "xtm_batch_create: null line");
+        if (lines[k]->device != device)
+            return fail(XTM_ERR_ARG, "xtm_batch_create: lines on different devices");
+        if (lines[k]->num_elements > 0x7fffffff)
+            return fail(XTM_ERR_ARG, "xtm_batch_create: line too long");
+    }
+    xtm_batch *B = new xtm_batch();
+    B->device = device;
+    B->lines.assign(lines, lines + num_lines);
+    std::vector<const int64_t *> entries(num_lines);
+    std::vector<const double *> progs(num_lines);
+    std::vector<double> lengths(num_lines);
+    for (int32_t k = 0; k < num_lines; k++) {
+        B->nele.push_back(lines[k]->num_elements);
+        entries[k] = lines[k]->d_entry;
+        progs[k] = lines[k]->d_prog;
+        lengths[k] = lines[k]->line_length;
+    }
+    if (device < 0) {
+        *out = B;
+        return XTM_OK;
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        xtm_batch_destroy(B);
+        return hip_fail(e, "hipSetDevice");
+    }
+    const size_t n = (size_t)num_lines;
+    if ((e = hipMalloc(&B->d_entries, n * sizeof(int64_t *))) != hipSuccess ||
+        (e = hipMalloc(&B->d_progs, n * sizeof(double *))) != hipSuccess ||
+        (e = hipMalloc(&B->d_nele, n * sizeof(int64_t))) != hipSuccess ||
+        (e = hipMalloc(&B->d_lengths, n * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&B->d_p, sizeof(xtm_particles))) != hipSuccess ||
+        (e = hipMalloc(&B->d_m, sizeof(xtm_monitor))) != hipSuccess) {
+        xtm_batch_destroy(B);
+        return hip_fail(e, "hipMalloc");
+    }
+    if ((e = hipMemcpy(B->d_entries, entries.data(), n * sizeof(int64_t *),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(B->d_progs, progs.data(), n * sizeof(double *),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(B->d_nele, B->nele.data(), n * sizeof(int64_t),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(B->d_lengths, lengths.data(), n * sizeof(double),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        xtm_batch_destroy(B);
+        return hip_fail(e, "hipMemcpy");
+    }
+    *out = B;
+    return XTM_OK;
+}
+
+int32_t xtm_batch_set_layout(xtm_batch *B, const int64_t *line_start, int32_t block_size,
+                             int32_t waves_per_eu) {
+    if (!B || !line_start) return fail(XTM_ERR_ARG, "xtm_batch_set_layout: null argument");
+    if (block_size != 256 || waves_per_eu < 3 || waves_per_eu > 5)
+        return fail(XTM_ERR_ARG, "xtm_batch_set_layout: no kernel for this block size and "
+                                 "occupancy (256 threads, 3 to 5 waves per SIMD)");
+    const size_t nl = B->lines.size();
+    if (line_start[0] != 0) return fail(XTM_ERR_ARG, "xtm_batch_set_layout: line_start[0] != 0");
+    std::vector<BatchBlock> blocks;
+    for (size_t k = 0; k < nl; k++) {
+        if (line_start[k + 1] < line_start[k])
+            return fail(XTM_ERR_ARG, "xtm_batch_set_layout: line_start decreases");
+        for (int64_t f = line_start[k]; f < line_start[k + 1]; f += block_size) {
+            BatchBlock bb;
+            bb.first = f;
+            bb.line = (int32_t)k;
+            bb.count = (int32_t)std::min<int64_t>(block_size, line_start[k + 1] - f);
+            blocks.push_back(bb);
+        }
+    }
+    if (blocks.size() > 0x7fffffffu) return fail(XTM_ERR_ARG, "xtm_batch_set_layout: too many blocks");
+    // longest lines first: the last blocks to start are the short ones
+    std::stable_sort(blocks.begin(), blocks.end(), [&](const BatchBlock &a, const BatchBlock &b) {
+        return B->nele[a.line] > B->nele[b.line];
+    });
+    if (B->device >= 0 && !blocks.empty()) {
+        hipError_t e = hipSetDevice(B->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if ((int64_t)blocks.size() > B->d_blocks_cap) {
+            if (B->d_blocks) (void)hipFree(B->d_blocks);
+            B->d_blocks = nullptr;
+            B->d_blocks_cap = 0;
+            if ((e = hipMalloc(&B->d_blocks, blocks.size() * sizeof(BatchBlock))) != hipSuccess)
+                return hip_fail(e, "hipMalloc");
+            B->d_blocks_cap = (int64_t)blocks.size();
+        }
+        if ((e = hipMemcpy(B->d_blocks, blocks.data(), blocks.size() * sizeof(BatchBlock),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, "hipMemcpy");
+    }
+    B->line_start.assign(line_start, line_start + nl + 1);
+    B->blocks = std::move(blocks);
+    B->block_size = block_size;
+    B->waves_per_eu = waves_per_eu;
+    return XTM_OK;
+}
+
+int64_t xtm_batch_num_blocks(const xtm_batch *B) { return B ? (int64_t)B->blocks.size() : -1; }
+
+int32_t xtm_batch_copy_blocks(const xtm_batch *B, int64_t *out) {
+    if (!B || !out) return fail(XTM_ERR_ARG, "xtm_batch_copy_blocks: null argument");
+    for (size_t b = 0; b < B->blocks.size(); b++) {
+        out[3 * b] = B->blocks[b].line;
+        out[3 * b + 1] = B->blocks[b].first;
+        out[3 * b + 2] = B->blocks[b].count;
+    }
+    return XTM_OK;
+}
+
+int32_t xtm_batch_track(xtm_batch *B, const xtm_particles *P, int32_t num_turns,
+                        uint64_t track_flags, void *stream) {
+    if (!B || !P) return fail(XTM_ERR_ARG, "xtm_batch_track: null batch or particles");
+    if (B->device < 0) return fail(XTM_ERR_ARG, "xtm_batch_track: batch was created without a device");
+    if (B->line_start.empty()) return fail(XTM_ERR_ARG, "xtm_batch_track: no layout set");
+    if (num_turns < 0) return fail(XTM_ERR_ARG, "xtm_batch_track: negative number of turns");
+    if (P->capacity < B->line_start.back())
+        return fail(XTM_ERR_ARG, "xtm_batch_track: particle set smaller than the layout");
+    if ((track_flags >> XTM_FLAG_SR_TAPER) & 1)
+        return fail(XTM_ERR_UNSUPPORTED, "SR tapering is not supported");
+    if ((track_flags >> XTM_FLAG_BACKTRACK) & 1)
+        return fail(XTM_ERR_UNSUPPORTED, "xtm_batch_track: backtracking is not supported");
+    if (num_turns == 0 || B->blocks.empty()) return XTM_OK;
+    hipError_t e = hipSetDevice(B->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    xtm_monitor M;
+    std::memset(&M, 0, sizeof(M));
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, s, *P, M, B->d_p, B->d_m);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "args_kernel launch");
+    const dim3 grid((unsigned)B->blocks.size());
+    const int64_t *const *entries = B->d_entries;
+    const double *const *progs = B->d_progs;
+    if (B->waves_per_eu == 5)
+        hipLaunchKernelGGL((batch_kernel<256, 5>), grid, dim3(256), 0, s, B->d_blocks, entries,
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+    else if (B->waves_per_eu == 4)
+        hipLaunchKernelGGL((batch_kernel<256, 4>), grid, dim3(256), 0, s, B->d_blocks, entries,
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+    else
+        hipLaunchKernelGGL((batch_kernel<256, 3>), grid, dim3(256), 0, s, B->d_blocks, entries,
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch_kernel launch");
     return XTM_OK;
 }
 

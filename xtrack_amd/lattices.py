@@ -75,6 +75,23 @@ def collider_ring(n_arcs=8, n_cells=23, k2f=0.0, k2d=0.0, gap=1.3, n_gap_pieces=
     return line
 
 
+def collider_variant(seed, k2f=0.0, k2d=0.0, n_arcs=8, cells_per_arc=(8, 40), rel_err=1e-4,
+                     **kwargs):
+    """Lattice variant of the collider ring for batched studies: the number
+    of cells per arc drawn uniformly in `cells_per_arc` (inclusive) and every
+    quadrupole k1 and sextupole k2 multiplied by 1 + N(0, rel_err), all from
+    numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    n_cells = int(rng.integers(cells_per_arc[0], cells_per_arc[1] + 1))
+    line = collider_ring(n_arcs=n_arcs, n_cells=n_cells, k2f=k2f, k2d=k2d, **kwargs)
+    for ee in line.elements:
+        if isinstance(ee, xe.Quadrupole):
+            ee.k1 = ee.k1 * (1.0 + rel_err * rng.standard_normal())
+        elif isinstance(ee, xe.Sextupole):
+            ee.k2 = ee.k2 * (1.0 + rel_err * rng.standard_normal())
+    return line
+
+
 def set_sextupoles(line, k2f, k2d):
     for nn, ee in zip(line.element_names, line.elements):
         if nn.startswith('sf.'):
