@@ -73,6 +73,13 @@ def parse():
     ap.add_argument('--waves-per-eu', type=int, default=0,
                     help='occupancy target of the compiled kernel (0: library default)')
     ap.add_argument('--rematch', action='store_true')
+    ap.add_argument('--workload', choices=('collider', 'batch'), default='collider',
+                    help='collider: the thick ring (the headline line); batch: 64 lattice '
+                         'variants of it with uneven particle counts in one launch')
+    ap.add_argument('--batch-variants', type=int, default=64)
+    ap.add_argument('--batch-max-particles', type=int, default=200_000)
+    ap.add_argument('--batch-graph', action='store_true',
+                    help='replay each step from a captured HIP graph')
     return ap.parse_args()
 
 
@@ -165,6 +172,85 @@ def cpu_baseline(line_fn, tw, n, turns):
                       f'reference tracker with platform libm, OpenMP {threads} threads'}
 
 
+def main_batch(args):
+    """Batched lattice variants: `--batch-variants` variants of the ring
+    (seeds 0.., cells per arc uniform in 8..40, quadrupole and sextupole
+    strengths scattered by N(0, 1e-4) relative), particle counts log-uniform
+    in [1000, --batch-max-particles], all tracked in one launch per step of
+    `--turns-per-step` turns.  Beams: Gaussian, the same small emittance for
+    every variant (not matched to each variant's optics)."""
+    import torch
+    import xtrack_amd as xt
+    from xtrack_amd import lattices, _native
+    rng = np.random.default_rng(12345)
+    nv = args.batch_variants
+    counts = np.exp(rng.uniform(np.log(1000), np.log(args.batch_max_particles), nv))
+    counts = counts.astype(np.int64)
+    t0 = time.perf_counter()
+    lines = [lattices.collider_variant(s, k2f=K2F, k2d=K2D) for s in range(nv)]
+    parts = []
+    for k, n in enumerate(counts):
+        r = np.random.default_rng(1000 + k)
+        parts.append(xt.Particles(p0c=7e12, mass0=xt.PROTON_MASS_EV,
+                                  x=r.normal(0, 3e-4, n), px=r.normal(0, 3e-6, n),
+                                  y=r.normal(0, 3e-4, n), py=r.normal(0, 3e-6, n),
+                                  zeta=r.normal(0, SIGMA_Z, n),
+                                  delta=r.normal(0, SIGMA_DELTA, n)))
+    bt = xt.BatchTracker(lines, device='cuda:0')
+    setup_s = time.perf_counter() - t0
+    # operations per particle-turn of each variant (same counting as the ring)
+    lib = _native.load()
+    flop = np.zeros(nv)
+    coords = np.array([1e-5, 1e-7, 1e-5, 1e-7, 1e-3, 1e-5])
+    for k, h in enumerate(bt._handles):
+        ops = np.zeros(6)
+        _native.check(lib.xtm_line_count_ops(h.ptr, coords.ctypes.data, 7e12,
+                                             xt.PROTON_MASS_EV, 1.0, ops.ctypes.data))
+        flop[k] = ops[5] + FLOP_PER_CALL * ops[4]
+    cfg = bt.tune(parts, turns=1)
+    tps = args.turns_per_step
+    for _ in range(args.warmup):
+        bt.track(parts, num_turns=tps, turns_per_launch=tps, use_graph=args.batch_graph)
+    before = np.array([int(p.to_numpy()['at_turn'].sum()) for p in parts])
+    alive0 = np.array([int((p.to_numpy()['state'] > 0).sum()) for p in parts])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    info = bt.track(parts, num_turns=tps * args.steps, turns_per_launch=tps,
+                    use_graph=args.batch_graph)
+    elapsed = time.perf_counter() - t1
+    after = np.array([int(p.to_numpy()['at_turn'].sum()) for p in parts])
+    pturns = int((after - before).sum())
+    kern_s = info['kernel_ms'] / 1e3
+    achieved = float(np.sum(flop * (after - before))) / kern_s / 1e12
+    nele = [len(ln.elements) for ln in lines]
+    out = {
+        'metric': 'particle-turns/s (FP64) on 1 MI355X, batched lattice variants',
+        'value': pturns / kern_s, 'unit': 'particle-turns/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': info['kernel_ms'] / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic (seeded Gaussian beams, synthetic lattice variants)',
+        'config': {'workload': f'{nv} variants of the collider ring (cells per arc 8..40, '
+                               'quad/sextupole errors 1e-4), one launch per step',
+                   'variants': nv, 'elements_min_max': [min(nele), max(nele)],
+                   'particles_total': int(counts.sum()),
+                   'particles_min_max': [int(counts.min()), int(counts.max())],
+                   'turns_per_step': tps, 'kernel': 'element-program interpreter (batch)',
+                   'graph': bool(info['graph']), 'autotuned': list(cfg),
+                   'tune_ms': {f'{k[0]}x{k[1]}': round(v, 2)
+                               for k, v in bt.tune_timings.items()},
+                   'alive_start_end': [int(alive0.sum()),
+                                       int(sum((p.to_numpy()['state'] > 0).sum()
+                                               for p in parts))],
+                   'host_setup_s': round(setup_s, 1), 'wall_s': round(elapsed, 2)},
+        'roofline': {'bound': 'mfma', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
+                                                      '78.6 TFLOP/s, vector and matrix)',
+                     'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -172,6 +258,9 @@ def main():
               'particles shard across GPUs with no exchange during tracking, '
               'so N GPUs give N times the per-GPU rate.', file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'batch':
+        main_batch(args)
+        return
     traffic = None
     if args.traffic == 'measure':
         try:
