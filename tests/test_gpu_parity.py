@@ -336,3 +336,34 @@ def test_segment_runs_bit_exact():
             for ff in ('x', 'px', 'y', 'py', 'zeta', 'at_element', 'state'):
                 assert np.array_equal(getattr(lg.record_last_track, ff),
                                       getattr(lc.record_last_track, ff), equal_nan=True), ff
+
+
+def test_da_scan_survival_turns():
+    """A dynamic-aperture scan on a sextupole-perturbed ring: the GPU
+    tracker gives the oracle's survival turn for every grid particle."""
+    from xtrack_amd import lattices, da
+    print('[da scan]', flush=True)
+
+    def ring():
+        line = lattices.thin_fodo_ring()
+        els = list(line.elements)
+        names = list(line.element_names)
+        for k in range(0, 100, 10):
+            i = names.index(f'q{k}')
+            els.insert(i + 1, xt.Multipole(knl=[0.0, 0.0, 8.0 if k % 20 else -5.0]))
+            names.insert(i + 1, f'sx{k}')
+        out = xt.Line(elements=els, element_names=names)
+        out.particle_ref = line.particle_ref
+        return out
+
+    lg, lc = ring(), ring()
+    lg.build_tracker(device='cuda:0', compile=_MODE['compile'])
+    lc.build_tracker(backend=OracleBackend())
+    kw = dict(r_max=12, n_r=24, n_angles=5, num_turns=200, nemitt_x=2.5e-6, nemitt_y=2.5e-6,
+              deltas=(0.0, 2e-3))
+    rg, rc = da.da_scan(lg, **kw), da.da_scan(lc, **kw)
+    assert np.array_equal(rg['survival'], rc['survival'])
+    assert rg['da'] == rc['da']
+    surv = rc['survival']
+    assert 0 < (surv < 200).sum() < len(surv)
+    assert len(np.unique(surv)) > 5
