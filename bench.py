@@ -80,6 +80,9 @@ def parse():
     ap.add_argument('--batch-max-particles', type=int, default=200_000)
     ap.add_argument('--batch-graph', action='store_true',
                     help='replay each step from a captured HIP graph')
+    ap.add_argument('--batch-compile', action='store_true',
+                    help='compile the variants into one kernel (element structures with '
+                         'constants read from each line) instead of the interpreter')
     return ap.parse_args()
 
 
@@ -196,7 +199,7 @@ def main_batch(args):
                                   y=r.normal(0, 3e-4, n), py=r.normal(0, 3e-6, n),
                                   zeta=r.normal(0, SIGMA_Z, n),
                                   delta=r.normal(0, SIGMA_DELTA, n)))
-    bt = xt.BatchTracker(lines, device='cuda:0')
+    bt = xt.BatchTracker(lines, device='cuda:0', compile=args.batch_compile)
     setup_s = time.perf_counter() - t0
     # operations per particle-turn of each variant (same counting as the ring)
     lib = _native.load()
@@ -235,7 +238,9 @@ def main_batch(args):
                    'variants': nv, 'elements_min_max': [min(nele), max(nele)],
                    'particles_total': int(counts.sum()),
                    'particles_min_max': [int(counts.min()), int(counts.max())],
-                   'turns_per_step': tps, 'kernel': 'element-program interpreter (batch)',
+                   'turns_per_step': tps,
+                   'kernel': ('compiled batch code (element structures)' if bt.is_compiled
+                              else 'element-program interpreter (batch)'),
                    'graph': bool(info['graph']), 'autotuned': list(cfg),
                    'tune_ms': {f'{k[0]}x{k[1]}': round(v, 2)
                                for k, v in bt.tune_timings.items()},

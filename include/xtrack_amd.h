@@ -344,6 +344,22 @@ int32_t xtm_batch_set_layout(xtm_batch *batch, const int64_t *line_start,
 int64_t xtm_batch_num_blocks(const xtm_batch *batch);
 int32_t xtm_batch_copy_blocks(const xtm_batch *batch, int64_t *out);
 
+/* Compile the batch's lines into one kernel (hiprtc, gfx950): one function
+ * per distinct element structure, with the constants read from each line's
+ * program, so that variants differing only in strengths share the code.
+ * Particles with chi == 1 then track on it; the interpreter follows on the
+ * same stream for the others.  Returns XTM_ERR_UNSUPPORTED when the lines
+ * have more than 2,048 distinct structures. */
+int32_t xtm_batch_specialize(xtm_batch *batch, int32_t waves_per_eu);
+int32_t xtm_batch_is_specialized(const xtm_batch *batch);
+
+/* Diagnostics: the source xtm_batch_specialize compiles (as
+ * xtm_line_generate_source) and the structure index of every element of
+ * every line, concatenated in line order (recs may be NULL).  Works on
+ * host-only batches; the CPU tests compile the source for the host. */
+int32_t xtm_batch_generate_source(const xtm_batch *batch, int32_t waves_per_eu, char *buf,
+                                  int64_t cap, int64_t *len, int32_t *recs);
+
 /* Track num_turns full turns, asynchronously on `stream`.  The particle set
  * must hold at least line_start[num_lines] particles.  Launches kernels
  * only (no allocation, no copy), so it may be captured in a HIP graph. */

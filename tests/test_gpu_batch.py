@@ -67,12 +67,14 @@ def _same_species(parts):
     return parts
 
 
-def test_batch_bit_exact_uneven_variants():
-    print('[batch] uneven variants, turn blocks', flush=True)
+@pytest.mark.parametrize('compiled', [True, False], ids=['compiled', 'interpreter'])
+def test_batch_bit_exact_uneven_variants(compiled):
+    print(f'[batch] uneven variants, turn blocks, compiled={compiled}', flush=True)
     lines = _variants()
     parts = _same_species(_particles())
     ref = _oracle(_variants(), [p.copy() for p in parts], 5)
-    bt = xt.BatchTracker(lines, device='cuda:0')
+    bt = xt.BatchTracker(lines, device='cuda:0', compile=compiled)
+    assert bt.is_compiled == compiled
     got = [p.copy() for p in parts]
     info = bt.track(got, num_turns=5, turns_per_launch=2)
     assert info['launches'] == 3

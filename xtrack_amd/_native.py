@@ -149,6 +149,13 @@ def load():
     lib.xtm_batch_track.restype = c.c_int32
     lib.xtm_batch_track.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
                                     c.c_uint64, c.c_void_p]
+    lib.xtm_batch_specialize.restype = c.c_int32
+    lib.xtm_batch_specialize.argtypes = [c.c_void_p, c.c_int32]
+    lib.xtm_batch_is_specialized.restype = c.c_int32
+    lib.xtm_batch_is_specialized.argtypes = [c.c_void_p]
+    lib.xtm_batch_generate_source.restype = c.c_int32
+    lib.xtm_batch_generate_source.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64,
+                                              c.POINTER(c.c_int64), c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

@@ -4,22 +4,29 @@ The reference tracks one line per ``Line.track`` call, one ``track_line``
 launch per call (xtrack/tracker.py:1372); a study over lattice variants
 (error seeds, strength scans) loops over lines.  ``BatchTracker`` puts the
 particles of all variants into one particle set, each variant's particles
-in a contiguous range, and tracks them in one launch of the element-program
-interpreter: every 256-thread block holds particles of one variant only,
+in a contiguous range, and tracks them in one launch: every 256-thread
+block holds particles of one variant only,
 so element dispatch stays uniform within a wave, and blocks of longer
 lines are dispatched first.  Each variant's particles end exactly as
 ``line.track(particles, num_turns=n)`` leaves them (full turns from
 element 0, s reset at the end of each turn, no monitor).
 
 Variants whose quadrupoles and sextupoles carry individual errors have as
-many distinct element records as elements, so they run on the interpreter
-rather than on per-line compiled code.
+many distinct element records as elements, too many for per-line literal
+code.  With ``compile=True`` the batch is compiled once for all its lines
+with one function per distinct element *structure* (op sequence, with the
+constants read from each line's program), so that every variant runs the
+same compiled code; particles with chi != 1 follow on the interpreter.
+On the 64-variant ring study this is no faster than the interpreter (the
+constants come from memory through the scalar cache either way, and the
+variants' programs compete for it), so the interpreter is the default.
 
 Turns run in launches of ``turns_per_launch``; with ``use_graph`` one such
 launch is captured in a HIP graph (torch.cuda.CUDAGraph) and replayed.  The
-autotuner (``tune``) times one turn of each kernel instance (occupancy 3, 4
-or 5 waves per SIMD at 256 threads per block) on a copy of the particles and
-keeps the fastest.
+autotuner (``tune``) times one turn of each interpreter kernel instance
+(occupancy 3, 4 or 5 waves per SIMD at 256 threads per block) on a copy of
+the particles and keeps the fastest; the compiled batch code is built for 5
+waves per SIMD.
 """
 import ctypes
 
@@ -35,7 +42,7 @@ CANDIDATES = ((BLOCK, 3), (BLOCK, 4), (BLOCK, 5))
 
 
 class BatchTracker:
-    def __init__(self, lines, device='cuda:0'):
+    def __init__(self, lines, device='cuda:0', compile=False):
         import torch
         self.lib = _native.load()
         if self.lib.xtm_device_count() < 1:
@@ -60,9 +67,20 @@ class BatchTracker:
         self._batch = ctypes.c_void_p()
         _native.check(self.lib.xtm_batch_create(arr, len(self._handles),
                                                 ctypes.byref(self._batch)))
+        if compile not in ('auto', True, False):
+            raise ValueError("compile must be 'auto', True or False")
+        if compile is not False:
+            st = self.lib.xtm_batch_specialize(self._batch, 0)
+            # too many distinct element structures: 'auto' keeps the interpreter
+            if st != 0 and not (st == _native.XTM_ERR_UNSUPPORTED and compile == 'auto'):
+                _native.check(st)
         self.config = None
         self.tune_timings = None
         self.last_track_info = None
+
+    @property
+    def is_compiled(self):
+        return bool(self.lib.xtm_batch_is_specialized(self._batch))
 
     def __del__(self):
         try:
@@ -186,6 +204,7 @@ class BatchTracker:
         ev1.record()
         torch.cuda.synchronize(self.device)
         self.last_track_info = {'kernel_ms': ev0.elapsed_time(ev1), 'config': self.config,
+                                'compiled': self.is_compiled,
                                 'launches': n_full + (1 if rest else 0),
                                 'graph': graph is not None,
                                 'particles': int(starts[-1])}

@@ -67,6 +67,12 @@ struct Gen {
         if (loads) return "rp[" + std::to_string(i - base) + "]";
         return lit(prog[i]);
     }
+    // address of program word i: relative to the record in loads mode, so
+    // that records of the same structure produce the same text
+    std::string addr(int64_t i) const {
+        if (loads) return "rp + " + std::to_string(i - base);
+        return "prog + " + std::to_string(i);
+    }
     std::string cs(int64_t i, int n) const {
         std::string s;
         for (int k = 0; k < n; k++) s += ", " + c(i + k);
@@ -105,7 +111,7 @@ struct Gen {
         const int64_t mask = wbits(a[2]);
         if (!in_range(desc, KD_ARR)) return false;
         line("if (!xt::kick_sane(p)) {");
-        line("    xt::kick_literal_apply(p, " + w + ", prog + " + std::to_string(desc) + ");");
+        line("    xt::kick_literal_apply(p, " + w + ", " + addr(desc) + ");");
         line("} else {");
         line("    const double x = p.x, y = p.y;");
         for (int k = 0; k < 3; k++) {
@@ -130,7 +136,7 @@ struct Gen {
             const int n = kNargs[op];
             if (!in_range(pc + 1, n)) return false;
             const int64_t a = pc + 1;
-            const std::string at = "prog + " + std::to_string(pc + 1);
+            const std::string at = addr(pc + 1);
             cost += 1;
             if (op == OP_POLAR || op == OP_DIPQUAD || op == OP_YOSHIDA4_BEND ||
                 op == OP_YOSHIDA6_BEND || op == OP_RF_KICK || op == OP_X_ROT || op == OP_Y_ROT ||
@@ -166,7 +172,9 @@ struct Gen {
             case OP_Y_ROT: line("xt::op_y_rot(p, m" + cs(a, 3) + ");"); break;
             case OP_S_ROT: line("xt::op_s_rot(p" + cs(a, 2) + ");"); break;
             case OP_RF_MULT:
-                line("xt::op_rf_mult(p, m, " + at + ", prog, c.q0, c.kill_cavity());");
+                // the descriptor holds program offsets: absolute addressing
+                line("xt::op_rf_mult(p, m, prog + " + std::to_string(pc + 1) +
+                     ", prog, c.q0, c.kill_cavity());");
                 break;
             default: return false;
             }
@@ -365,6 +373,127 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "                                xt::ProgramRunner{entry, prog}, c, num_turns,\n"
          "                                ele_start, num_ele_track, flag_end_turn_actions,\n"
          "                                flag_reset_s, flag_monitor, mon);\n"
+         "}\n"
+         "#endif\n";
+    src = std::move(s);
+    return XTM_OK;
+}
+
+int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &entries,
+                             const std::vector<const std::vector<double> *> &progs,
+                             int32_t waves_per_eu, std::string &src,
+                             std::vector<std::vector<int32_t>> &recs, std::string &err) {
+    // one function per distinct structure: the record code of loads mode
+    // (constants read at rp = prog + entry[e]), deduplicated by its text
+    std::map<std::string, int32_t> by_text;
+    std::vector<std::string> bodies;
+    recs.assign(entries.size(), {});
+    for (size_t l = 0; l < entries.size(); l++) {
+        const std::vector<int64_t> &entry = *entries[l];
+        const std::vector<double> &prog = *progs[l];
+        std::map<int64_t, int32_t> of_pc;
+        recs[l].resize(entry.size());
+        for (size_t e = 0; e < entry.size(); e++) {
+            auto it = of_pc.find(entry[e]);
+            if (it == of_pc.end()) {
+                Gen g(prog, entry[e], true);
+                if (!g.record(entry[e])) {
+                    err = "cannot generate code for line " + std::to_string(l) +
+                          " at program offset " + std::to_string(entry[e]);
+                    return XTM_ERR_ARG;
+                }
+                auto ins = by_text.emplace(g.out, (int32_t)bodies.size());
+                if (ins.second) {
+                    bodies.push_back(g.out);
+                    if ((int64_t)bodies.size() > XT_CODEGEN_MAX_RECORDS) {
+                        err = "the lines have more than " +
+                              std::to_string(XT_CODEGEN_MAX_RECORDS) + " distinct element structures";
+                        return XTM_ERR_UNSUPPORTED;
+                    }
+                }
+                it = of_pc.emplace(entry[e], ins.first->second).first;
+            }
+            recs[l][e] = it->second;
+        }
+    }
+    std::string s;
+    s += "// generated batch code: element structures as straight-line calls, constants\n"
+         "// read from each line's program\n";
+    s += "#if defined(__HIPCC__) && !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
+    s += "#include \"xt_ops.h\"\n\n";
+    s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";
+    s += "#else\n#define XT_GEN_FN static inline\n#endif\n\n";
+    s += "namespace xt_gen {\n";
+    for (size_t r = 0; r < bodies.size(); r++) {
+        s += "XT_GEN_FN void rec_" + std::to_string(r) +
+             "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
+             "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n"
+             "    (void)m; (void)prog; (void)rp; (void)c;\n";
+        s += bodies[r];
+        s += "}\n\n";
+    }
+    s += "struct Runner {\n"
+         "    const int32_t *__restrict__ rec;\n"
+         "    const int64_t *__restrict__ entry;\n"
+         "    const double *__restrict__ prog;\n"
+         "    struct Key { int32_t r; int64_t pc; };\n"
+         "    static constexpr bool kSegments = false;\n"
+         "    XT_HD static int32_t span(Key) { return 0; }\n"
+         "    XT_HD int32_t segment(xt::PState<double> &, const xt::PRef &, Key,\n"
+         "                          const xt::TrackCtx &) const { return 0; }\n"
+         "    XT_HD Key key(int64_t e) const { return Key{rec[e], entry[e]}; }\n"
+         "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
+         "                          const xt::TrackCtx &c) const {\n"
+         "        const double *__restrict__ rp = prog + k.pc;\n"
+         "        switch (k.r) {\n";
+    for (size_t r = 0; r < bodies.size(); r++)
+        s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
+             "(p, m, prog, rp, c); break;\n";
+    s += "        default: p.state = -999; break;\n"
+         "        }\n"
+         "    }\n"
+         "};\n"
+         "/* one block of a batch: particles [first, first + count) of line `line` */\n"
+         "struct BatchBlock { int64_t first; int32_t line; int32_t count; };\n"
+         "}  // namespace xt_gen\n\n";
+    char occ[128];
+    std::snprintf(occ, sizeof occ, "__attribute__((amdgpu_waves_per_eu(%d)))",
+                  waves_per_eu > 0 ? waves_per_eu : 1);
+    s += "#if defined(__HIPCC__)\n";
+    s += std::string("extern \"C\" __global__ __launch_bounds__(256) ") +
+         (waves_per_eu > 0 ? occ : "") +
+         "\nvoid xtm_batch_line_kernel(const xt_gen::BatchBlock *__restrict__ blocks,\n"
+         "    const int32_t *const *__restrict__ recs, const int64_t *const *__restrict__ entries,\n"
+         "    const double *const *__restrict__ progs, const int64_t *__restrict__ nele,\n"
+         "    const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,\n"
+         "    int32_t num_turns, uint64_t flags) {\n"
+         "    const xt_gen::BatchBlock b = blocks[blockIdx.x];\n"
+         "    if ((int32_t)threadIdx.x >= b.count) return;\n"
+         "    const xtm_particles &P = *Pd;\n"
+         "    const int64_t i = b.first + threadIdx.x;\n"
+         "    const int64_t ne = nele[b.line];\n"
+         "    const xt::TrackCtx c{flags, lengths[b.line], P.q0, P.t_sim, ne};\n"
+         "    // chi == 1 particles; the interpreter batch kernel tracks the others\n"
+         "    xt::track_particle_chi1(P, i, xt_gen::Runner{recs[b.line], entries[b.line],\n"
+         "                            progs[b.line]}, c, num_turns, 0, (int32_t)ne, 1, 1, 0,\n"
+         "                            xt::MonRef{nullptr, Pd});\n"
+         "}\n"
+         "#else\n"
+         "// host build (tests): the same per-particle body over the blocks\n"
+         "extern \"C\" void xtm_batch_host(const xt_gen::BatchBlock *blocks, int64_t num_blocks,\n"
+         "    const int32_t *const *recs, const int64_t *const *entries, const double *const *progs,\n"
+         "    const int64_t *nele, const double *lengths, const xtm_particles *P,\n"
+         "    int32_t num_turns, uint64_t flags) {\n"
+         "    for (int64_t bi = 0; bi < num_blocks; bi++) {\n"
+         "        const xt_gen::BatchBlock b = blocks[bi];\n"
+         "        const int64_t ne = nele[b.line];\n"
+         "        const xt::TrackCtx c{flags, lengths[b.line], P->q0, P->t_sim, ne};\n"
+         "        for (int32_t t = 0; t < b.count; t++)\n"
+         "            xt::track_particle_with<false>(*P, b.first + t,\n"
+         "                xt_gen::Runner{recs[b.line], entries[b.line], progs[b.line]},\n"
+         "                xt::ProgramRunner{entries[b.line], progs[b.line]}, c, num_turns, 0,\n"
+         "                (int32_t)ne, 1, 1, 0, xt::MonRef{nullptr, P});\n"
+         "    }\n"
          "}\n"
          "#endif\n";
     src = std::move(s);

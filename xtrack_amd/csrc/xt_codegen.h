@@ -30,4 +30,16 @@ static_assert(XT_CODEGEN_MAX_RECORDS <= XT_KEY_REC_MASK + 1, "record index must 
 int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
                             int32_t waves_per_eu, std::string &src,
                             std::vector<int32_t> &rec_of_elem, std::string &err);
+
+// Source of a translation unit that tracks a batch of lines in one launch:
+// one function per distinct element structure (the record code with its
+// constants read from the line's program at rp = prog + entry[e], so that
+// the records of all lines that differ only in their constants share one
+// function), an element runner keyed by (structure, program offset), and
+// the kernel `xtm_batch_line_kernel` over a block table (one line per
+// block).  recs[l][e] receives the structure index of element e of line l.
+int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &entries,
+                             const std::vector<const std::vector<double> *> &progs,
+                             int32_t waves_per_eu, std::string &src,
+                             std::vector<std::vector<int32_t>> &recs, std::string &err);
 #endif

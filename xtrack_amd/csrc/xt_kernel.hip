@@ -90,12 +90,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const BatchBlock *__restrict__ blocks, const int64_t *const *__restrict__ entries,
     const double *const *__restrict__ progs, const int64_t *__restrict__ nele,
     const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,
-    int32_t num_turns, uint64_t flags) {
+    int32_t num_turns, uint64_t flags, int32_t skip_chi1) {
     static_assert(BS <= BLOCK, "the per-thread LDS slots are laid out for 256 threads");
     const BatchBlock b = blocks[blockIdx.x];
     if ((int32_t)threadIdx.x >= b.count) return;
     const xtm_particles &P = *Pd;
     const int64_t i = b.first + threadIdx.x;
+    // after the compiled batch kernel: only the particles it left (chi != 1)
+    if (skip_chi1 && P.chi[i] == 1.0) return;
     const int64_t ne = nele[b.line];
     const xt::TrackCtx c{flags, lengths[b.line], P.q0, P.t_sim, ne};
     const xt::ProgramRunner run{entries[b.line], progs[b.line]};
@@ -186,6 +188,11 @@ struct xtm_batch {
     int64_t d_blocks_cap = 0;
     xtm_particles *d_p = nullptr;
     xtm_monitor *d_m = nullptr;
+    // generated batch code (xtm_batch_specialize)
+    std::vector<int32_t *> d_recs;
+    const int32_t **d_rec_ptrs = nullptr;
+    hipModule_t module = nullptr;
+    hipFunction_t kernel = nullptr;
 };
 
 #ifndef XTM_RTC_DEFINES
@@ -510,6 +517,10 @@ int32_t xtm_batch_destroy(xtm_batch *B) {
         if (B->d_blocks) (void)hipFree(B->d_blocks);
         if (B->d_p) (void)hipFree(B->d_p);
         if (B->d_m) (void)hipFree(B->d_m);
+        for (int32_t *r : B->d_recs)
+            if (r) (void)hipFree(r);
+        if (B->d_rec_ptrs) (void)hipFree(B->d_rec_ptrs);
+        if (B->module) (void)hipModuleUnload(B->module);
     }
     delete B;
     return XTM_OK;
@@ -656,19 +667,133 @@ int32_t xtm_batch_track(xtm_batch *B, const xtm_particles *P, int32_t num_turns,
     const dim3 grid((unsigned)B->blocks.size());
     const int64_t *const *entries = B->d_entries;
     const double *const *progs = B->d_progs;
+    int32_t skip = 0;
+    if (B->kernel) {
+        const void *blocks = B->d_blocks;
+        const int32_t *const *recs = B->d_rec_ptrs;
+        const int64_t *nele = B->d_nele;
+        const double *lengths = B->d_lengths;
+        const xtm_particles *Pd = B->d_p;
+        int32_t a_turns = num_turns;
+        uint64_t a_flags = track_flags;
+        void *args[] = {&blocks, &recs, &entries, &progs, &nele, &lengths, &Pd, &a_turns,
+                        &a_flags};
+        e = hipModuleLaunchKernel(B->kernel, grid.x, 1, 1, BLOCK, 1, 1, 0, s, args, nullptr);
+        if (e != hipSuccess) return hip_fail(e, "xtm_batch_line_kernel launch");
+        skip = 1;   // the interpreter follows for the particles with chi != 1
+    }
     if (B->waves_per_eu == 5)
         hipLaunchKernelGGL((batch_kernel<256, 5>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else if (B->waves_per_eu == 4)
         hipLaunchKernelGGL((batch_kernel<256, 4>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else
         hipLaunchKernelGGL((batch_kernel<256, 3>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch_kernel launch");
     return XTM_OK;
 }
+
+int32_t xtm_batch_generate_source(const xtm_batch *B, int32_t waves_per_eu, char *buf,
+                                  int64_t cap, int64_t *len, int32_t *recs) {
+    if (!B || !len) return fail(XTM_ERR_ARG, "xtm_batch_generate_source: null argument");
+    std::vector<const std::vector<int64_t> *> entries;
+    std::vector<const std::vector<double> *> progs;
+    for (const xtm_line *L : B->lines) {
+        entries.push_back(&L->entry);
+        progs.push_back(&L->prog);
+    }
+    std::string src, err;
+    std::vector<std::vector<int32_t>> rec;
+    int st = xt_generate_batch_source(entries, progs, waves_per_eu, src, rec, err);
+    if (st != XTM_OK) return fail(st, "xtm_batch_generate_source: " + err);
+    *len = (int64_t)src.size();
+    if (buf && cap > 0) {
+        const int64_t n = (int64_t)src.size() < cap - 1 ? (int64_t)src.size() : cap - 1;
+        std::memcpy(buf, src.data(), (size_t)n);
+        buf[n] = '\0';
+    }
+    if (recs) {
+        int64_t o = 0;
+        for (const auto &r : rec) {
+            if (!r.empty()) std::memcpy(recs + o, r.data(), r.size() * sizeof(int32_t));
+            o += (int64_t)r.size();
+        }
+    }
+    return XTM_OK;
+}
+
+int32_t xtm_batch_specialize(xtm_batch *B, int32_t waves_per_eu) {
+    if (!B) return fail(XTM_ERR_ARG, "xtm_batch_specialize: null batch");
+    if (B->device < 0)
+        return fail(XTM_ERR_ARG, "xtm_batch_specialize: batch was created without a device");
+    if (waves_per_eu < 0 || waves_per_eu > 8)
+        return fail(XTM_ERR_ARG, "xtm_batch_specialize: waves_per_eu outside 0..8");
+    if (waves_per_eu == 0) waves_per_eu = XTM_COMPILED_WAVES_PER_EU;
+    std::vector<const std::vector<int64_t> *> entries;
+    std::vector<const std::vector<double> *> progs;
+    for (const xtm_line *L : B->lines) {
+        entries.push_back(&L->entry);
+        progs.push_back(&L->prog);
+    }
+    std::string src, err;
+    std::vector<std::vector<int32_t>> rec;
+    int st = xt_generate_batch_source(entries, progs, waves_per_eu, src, rec, err);
+    if (st != XTM_OK) return fail(st, "xtm_batch_specialize: " + err);
+    std::vector<char> code;
+    st = rtc_compile(src, code);
+    if (st != XTM_OK) return st;
+    hipError_t e = hipSetDevice(B->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipModule_t mod = nullptr;
+    if ((e = hipModuleLoadData(&mod, code.data())) != hipSuccess)
+        return hip_fail(e, "hipModuleLoadData");
+    hipFunction_t fn = nullptr;
+    if ((e = hipModuleGetFunction(&fn, mod, "xtm_batch_line_kernel")) != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        return hip_fail(e, "hipModuleGetFunction");
+    }
+    std::vector<int32_t *> d_recs(rec.size(), nullptr);
+    const int32_t **d_ptrs = nullptr;
+    auto cleanup = [&]() {
+        for (int32_t *r : d_recs)
+            if (r) (void)hipFree(r);
+        if (d_ptrs) (void)hipFree(d_ptrs);
+        (void)hipModuleUnload(mod);
+    };
+    for (size_t l = 0; l < rec.size(); l++) {
+        const size_t n = rec[l].empty() ? 1 : rec[l].size();
+        if ((e = hipMalloc(&d_recs[l], n * sizeof(int32_t))) != hipSuccess) {
+            cleanup();
+            return hip_fail(e, "hipMalloc");
+        }
+        if (!rec[l].empty() &&
+            (e = hipMemcpy(d_recs[l], rec[l].data(), rec[l].size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice)) != hipSuccess) {
+            cleanup();
+            return hip_fail(e, "hipMemcpy");
+        }
+    }
+    if ((e = hipMalloc(&d_ptrs, rec.size() * sizeof(int32_t *))) != hipSuccess ||
+        (e = hipMemcpy(d_ptrs, d_recs.data(), rec.size() * sizeof(int32_t *),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "hipMalloc/hipMemcpy");
+    }
+    for (int32_t *r : B->d_recs)
+        if (r) (void)hipFree(r);
+    if (B->d_rec_ptrs) (void)hipFree(B->d_rec_ptrs);
+    if (B->module) (void)hipModuleUnload(B->module);
+    B->d_recs = std::move(d_recs);
+    B->d_rec_ptrs = d_ptrs;
+    B->module = mod;
+    B->kernel = fn;
+    return XTM_OK;
+}
+
+int32_t xtm_batch_is_specialized(const xtm_batch *B) { return (B && B->kernel) ? 1 : 0; }
 
 int32_t xtm_synchronize(void *stream) {
     hipError_t e = hipStreamSynchronize((hipStream_t)stream);
