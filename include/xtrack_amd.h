@@ -366,6 +366,19 @@ int32_t xtm_batch_generate_source(const xtm_batch *batch, int32_t waves_per_eu, 
 int32_t xtm_batch_track(xtm_batch *batch, const xtm_particles *particles,
                         int32_t num_turns, uint64_t track_flags, void *stream);
 
+/* ---- survival-turn surrogate network --------------------------------------
+ * No counterpart in the reference.  C[M x N] = op(A) op(B) on the matrix
+ * cores: BF16 operands (raw 16-bit patterns), FP32 accumulation, op(A)(m, k)
+ * = A[m*sam + k*sak], op(B)(k, n) = B[k*sbk + n*sbn] (strides give plain or
+ * transposed operands), then + bias[n] when bias is not NULL and ReLU when
+ * relu != 0.  C (FP32, row stride ldc) always; Cb, when not NULL, the same
+ * values rounded to BF16 (nearest even).  Device pointers; asynchronous on
+ * `stream`.  M up to 65535 * 64 rows. */
+int32_t xtm_gemm_bf16(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B,
+                      int64_t sbk, int64_t sbn, float *C, int64_t ldc, uint16_t *Cb,
+                      const float *bias, int32_t relu, int64_t M, int64_t N, int64_t K,
+                      void *stream);
+
 /* Wait for all work queued on `stream`. */
 int32_t xtm_synchronize(void *stream);
 

@@ -1,0 +1,98 @@
+"""The surrogate network's matrix-core products and gradients (xtm_gemm_bf16
+through xtrack_amd.surrogate) against CPU references in float64 with the
+same BF16 rounding points."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(a):
+    """float64 values of float32 `a` rounded to BF16 (nearest even)."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize('M,N,K,ta,tb', [(1, 1, 1, False, False), (70, 33, 17, False, False),
+                                         (130, 257, 16, True, False),
+                                         (64, 64, 300, False, True), (5, 100, 40, True, True)])
+def test_gemm_against_cpu(M, N, K, ta, tb):
+    import torch
+    from xtrack_amd import surrogate as S
+    print(f'[gemm] {M}x{N}x{K} ta={ta} tb={tb}', flush=True)
+    rng = np.random.default_rng(M * 1000 + N)
+    a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    b = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    bias = rng.normal(size=N).astype(np.float32)
+    dev = torch.device('cuda:0')
+    ab = S.to_bf16(torch.from_numpy(a).to(dev))
+    bb = S.to_bf16(torch.from_numpy(b).to(dev))
+    for relu, use_bias in ((False, False), (True, True)):
+        c, cb = S.gemm(ab, bb, ta, tb, bias=torch.from_numpy(bias).to(dev) if use_bias else None,
+                       relu=relu, want_bf16=True)
+        A, B = _bf16_round(a), _bf16_round(b)
+        ref = (A.T if ta else A) @ (B.T if tb else B)
+        if use_bias:
+            ref = ref + bias
+        if relu:
+            ref = np.maximum(ref, 0.0)
+        got = c.cpu().numpy().astype(np.float64)
+        scale = np.abs(A).max() * np.abs(B).max() * K + 1.0
+        assert np.max(np.abs(got - ref)) <= 1e-6 * scale, np.max(np.abs(got - ref))
+        assert np.array_equal(cb.cpu().numpy(), S.to_bf16(c).cpu().numpy())
+
+
+def test_mlp_gradients_against_cpu():
+    """Backward pass of the network (matrix-core products) against float64
+    backpropagation on the CPU with the same BF16 roundings of activations,
+    weights and output gradients."""
+    import torch
+    from xtrack_amd import surrogate as S
+    print('[mlp gradients]', flush=True)
+    net = S.MLP(sizes=(16, 64, 48, 32, 1), seed=3)
+    rng = np.random.default_rng(1)
+    x = torch.from_numpy(rng.normal(size=(200, 16)).astype(np.float32)).to('cuda:0')
+    y = torch.from_numpy(rng.normal(size=200).astype(np.float32)).to('cuda:0')
+    out = net.forward(x, keep=True)
+    dout = 2 * (out - y) / 200
+    gW, gb = net.backward(dout)
+    # CPU reference
+    W = [w.cpu().numpy() for w in net.W]
+    b = [v.cpu().numpy().astype(np.float64) for v in net.b]
+    h = [_bf16_round(x.cpu().numpy())]
+    zs = []
+    for li in range(4):
+        z = (h[-1] @ _bf16_round(W[li]) + b[li]).astype(np.float32)
+        if li < 3:
+            z = np.maximum(z, 0)
+            h.append(_bf16_round(z))
+        zs.append(z.astype(np.float64))
+    assert np.allclose(out.cpu().numpy(), zs[-1][:, 0], rtol=1e-5, atol=1e-5)
+    dz = (2 * (zs[-1][:, 0].astype(np.float32) - y.cpu().numpy()) / 200).reshape(-1, 1)
+    for li in range(3, -1, -1):
+        dzb = _bf16_round(dz.astype(np.float32))
+        gw_ref = h[li].T @ dzb
+        assert np.allclose(gW[li].cpu().numpy(), gw_ref, rtol=2e-3, atol=1e-5 * np.abs(gw_ref).max()), li
+        assert np.allclose(gb[li].cpu().numpy(), dz.sum(axis=0), rtol=1e-4, atol=1e-6), li
+        if li > 0:
+            dh = dzb @ _bf16_round(W[li]).T
+            dz = (dh * (zs[li - 1] > 0)).astype(np.float32)
+
+
+def test_mlp_fits_a_smooth_function():
+    """Training on the matrix cores lowers the loss of a smooth target by
+    orders of magnitude."""
+    import torch
+    from xtrack_amd import surrogate as S
+    print('[mlp fit]', flush=True)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(4000, 16)).astype(np.float32)
+    y = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2] + 0.5 * x[:, 3] ** 2).astype(np.float32)
+    dev = torch.device('cuda:0')
+    net = S.MLP(seed=1)
+    hist = net.fit(torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev), epochs=60,
+                   batch=256, lr=2e-3)
+    assert hist[-1] < 0.05 * hist[0], hist[::10]
+    pred = net.predict(torch.from_numpy(x[:500]).to(dev)).cpu().numpy()
+    assert np.sqrt(np.mean((pred - y[:500]) ** 2)) < 0.15 * np.std(y)

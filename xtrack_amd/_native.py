@@ -156,6 +156,10 @@ def load():
     lib.xtm_batch_generate_source.restype = c.c_int32
     lib.xtm_batch_generate_source.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64,
                                               c.POINTER(c.c_int64), c.c_void_p]
+    lib.xtm_gemm_bf16.restype = c.c_int32
+    lib.xtm_gemm_bf16.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_int64,
+                                  c.c_int64, c.c_void_p, c.c_int64, c.c_void_p, c.c_void_p,
+                                  c.c_int32, c.c_int64, c.c_int64, c.c_int64, c.c_void_p]
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32
