@@ -73,7 +73,10 @@ def parse():
     ap.add_argument('--waves-per-eu', type=int, default=0,
                     help='occupancy target of the compiled kernel (0: library default)')
     ap.add_argument('--rematch', action='store_true')
-    ap.add_argument('--workload', choices=('collider', 'batch'), default='collider',
+    ap.add_argument('--da-angles', type=int, default=90)
+    ap.add_argument('--da-amplitudes', type=int, default=1000)
+    ap.add_argument('--da-turns', type=int, default=10000)
+    ap.add_argument('--workload', choices=('collider', 'batch', 'da'), default='collider',
                     help='collider: the thick ring (the headline line); batch: 64 lattice '
                          'variants of it with uneven particle counts in one launch')
     ap.add_argument('--batch-variants', type=int, default=64)
@@ -256,6 +259,48 @@ def main_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def main_da(args):
+    """Dynamic-aperture scan, plain against surrogate-screened, on the thin
+    FODO ring with sextupoles: polar grid of --da-angles x --da-amplitudes
+    (0-15 sigma) x delta in {0, 2.7e-4} (180,000 particles by default),
+    --da-turns turns.  Reports both
+    wall times (inputs built on the host, tracking on the GPU) and the
+    aperture error of the screened scan."""
+    import torch
+    from xtrack_amd import da, lattices, surrogate
+    line = lattices.sextupole_fodo_ring()
+    line.build_tracker(device='cuda:0')
+    kw = dict(r_max=15.0, n_r=args.da_amplitudes, n_angles=args.da_angles,
+              num_turns=args.da_turns, nemitt_x=2.5e-6, nemitt_y=2.5e-6, deltas=(0.0, 2.7e-4))
+    # warm the tracker (code load) on a tiny scan
+    da.da_scan(line, **dict(kw, n_r=4, n_angles=2, num_turns=4))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain = da.da_scan(line, **kw)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = surrogate.surrogate_da_scan(line, feature_turns=256, train_fraction=0.05, band=1.0,
+                                      epochs=60, seed=0, **kw)
+    t_sur = time.perf_counter() - t0
+    errs = np.array([abs(res['da'][k] - plain['da'][k]) for k in plain['da']])
+    n = len(plain['r'])
+    out = {
+        'metric': 'DA-scan wall time, plain vs surrogate-screened, 1 MI355X',
+        'value': t_plain / t_sur, 'unit': 'speed-up (plain time / surrogate time)',
+        'n_gpus': 1, 'higher_is_better': True, 'vs_baseline': None, 'dtype': 'f64 tracking, '
+        'bf16/f32 network', 'data': 'synthetic (polar grid, thin FODO ring with sextupoles)',
+        'config': {'workload': f'DA scan, {args.da_angles} angles x {args.da_amplitudes} '
+                               f'amplitudes (0-15 sigma) x 2 deltas, {args.da_turns} turns',
+                   'particles': n, 'plain_s': round(t_plain, 3),
+                   'surrogate_s': round(t_sur, 3),
+                   'surrogate_parts_s': {k: round(v, 3) for k, v in res['time'].items()},
+                   'tracked_full_fraction': round(len(res['tracked_full']) / n, 4),
+                   'da_error_sigma': {'max': float(errs.max()), 'mean': float(errs.mean())},
+                   'da_mean_sigma_plain': float(np.mean(list(plain['da'].values())))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -265,6 +310,9 @@ def main():
         sys.exit(2)
     if args.workload == 'batch':
         main_batch(args)
+        return
+    if args.workload == 'da':
+        main_da(args)
         return
     traffic = None
     if args.traffic == 'measure':

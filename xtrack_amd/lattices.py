@@ -75,6 +75,21 @@ def collider_ring(n_arcs=8, n_cells=23, k2f=0.0, k2d=0.0, gap=1.3, n_gap_pieces=
     return line
 
 
+def sextupole_fodo_ring(k2f=8.0, k2d=-5.0, every=10):
+    """The thin FODO ring with a thin sextupole after every `every`-th
+    quadrupole (alternating k2f, k2d): a fast ring with a dynamic aperture
+    of a few sigma for scans and surrogate tests."""
+    line = thin_fodo_ring()
+    els, names = list(line.elements), list(line.element_names)
+    for k in range(0, 100, every):
+        i = names.index(f'q{k}')
+        els.insert(i + 1, xe.Multipole(knl=[0.0, 0.0, k2f if (k // every) % 2 else k2d]))
+        names.insert(i + 1, f'sx{k}')
+    out = Line(elements=els, element_names=names)
+    out.particle_ref = line.particle_ref
+    return out
+
+
 def collider_variant(seed, k2f=0.0, k2d=0.0, n_arcs=8, cells_per_arc=(8, 40), rel_err=1e-4,
                      **kwargs):
     """Lattice variant of the collider ring for batched studies: the number

@@ -168,3 +168,186 @@ class MLP:
         xn = (x - self.x_mean) / self.x_std
         out = [self.forward(xn[s:s + batch]) for s in range(0, len(xn), batch)]
         return torch.cat(out) * self.y_std + self.y_mean
+
+
+# ---------------------------------------------------------------------------
+# dynamic-aperture scan screened by the surrogate
+
+def _take(p, idx):
+    """Particles holding the slots `idx` (device index tensor) of `p`."""
+    from .particles import Particles
+    out = Particles.__new__(Particles)
+    out._capacity = int(len(idx))
+    out.q0, out.mass0, out.t_sim = p.q0, p.mass0, p.t_sim
+    out.start_tracking_at_element = p.start_tracking_at_element
+    torch = _t()
+
+    def take(vv):
+        if vv.dtype == torch.uint32:     # no unsigned indexing on the device
+            return vv.view(torch.int32)[idx].contiguous().view(torch.uint32)
+        return vv[idx].contiguous()
+
+    out._arrays = {nn: take(vv) for nn, vv in p._arrays.items()}
+    out._device = p._device
+    return out
+
+
+def features(mon, x_norm0, y_norm0, delta, invW4, sx, sy, n_turns):
+    """16 features per particle from its initial normalised position and
+    the turn-by-turn record of its first n_turns turns: r, theta, delta,
+    the initial actions, mean, spread and maximum of the normalised actions
+    over the turns (relative to the initial ones), the mean phase advance
+    per turn in each plane (tunes), the fraction of the turns survived."""
+    torch = _t()
+    X = torch.stack([mon.x, mon.px, mon.y, mon.py])            # 4 x N x T
+    st = mon.state
+    alive = (st > 0).to(torch.float64)
+    Wi = torch.as_tensor(invW4, dtype=torch.float64, device=X.device)
+    Z = torch.einsum('ij,jnt->int', Wi, X.to(torch.float64))
+    zx, zpx = Z[0] / sx, Z[1] / sx
+    zy, zpy = Z[2] / sy, Z[3] / sy
+    jx, jy = 0.5 * (zx * zx + zpx * zpx), 0.5 * (zy * zy + zpy * zpy)
+    nal = alive.sum(dim=1).clamp_min(1.0)
+    r0 = torch.sqrt(torch.as_tensor(x_norm0 ** 2 + y_norm0 ** 2, device=X.device))
+    th0 = torch.atan2(torch.as_tensor(y_norm0, device=X.device),
+                      torch.as_tensor(x_norm0, device=X.device))
+    j0x, j0y = jx[:, 0].clamp_min(1e-12), jy[:, 0].clamp_min(1e-12)
+    mjx = (jx * alive).sum(1) / nal
+    mjy = (jy * alive).sum(1) / nal
+    sjx = torch.sqrt(((jx - mjx[:, None]) ** 2 * alive).sum(1) / nal)
+    sjy = torch.sqrt(((jy - mjy[:, None]) ** 2 * alive).sum(1) / nal)
+    mxx = (jx * alive).amax(1)
+    mxy = (jy * alive).amax(1)
+
+    def tune(z, pz):
+        ph = torch.atan2(-pz, z)
+        d = torch.remainder(ph[:, 1:] - ph[:, :-1], 2 * math.pi)
+        a = alive[:, 1:]
+        return (d * a).sum(1) / a.sum(1).clamp_min(1.0) / (2 * math.pi)
+
+    f = torch.stack([r0, th0, torch.as_tensor(delta * 1e3, device=X.device),
+                     torch.log1p(j0x), torch.log1p(j0y),
+                     torch.log(mjx.clamp_min(1e-12) / j0x), torch.log(mjy.clamp_min(1e-12) / j0y),
+                     sjx / mjx.clamp_min(1e-12), sjy / mjy.clamp_min(1e-12),
+                     torch.log(mxx.clamp_min(1e-12) / j0x), torch.log(mxy.clamp_min(1e-12) / j0y),
+                     tune(zx, zpx), tune(zy, zpy), nal / n_turns,
+                     torch.cos(th0), torch.sin(th0)], dim=1)
+    return torch.nan_to_num(f, nan=0.0, posinf=50.0, neginf=-50.0).to(torch.float32)
+
+
+def surrogate_da_scan(line, r_max, n_r, n_angles, num_turns, nemitt_x, nemitt_y,
+                      deltas=(0.0,), feature_turns=256, train_fraction=0.15, band=1.5,
+                      epochs=150, seed=0, net=None, feature_chunk=200_000):
+    """DA scan screened by the surrogate; returns the survival turns used
+    for the aperture (exact where tracked, predicted elsewhere), the
+    per-angle apertures, which particles were tracked in full, timings and
+    the network.  The line must have a device tracker.
+
+    band: half-width, in beam sigmas, of the amplitude band around each
+    angle's predicted aperture whose particles are tracked to num_turns."""
+    torch = _t()
+    from . import da
+    from .monitor import ParticlesMonitor
+    from .twiss import (_reference, compute_one_turn_matrix_finite_differences,
+                        find_closed_orbit, get_linear_normal_form)
+    dev = line.tracker.backend.device
+    t_all = time.perf_counter()
+    r, th, d = da.polar_grid(r_max, n_r, n_angles, deltas)
+    xn, yn = r * np.cos(th), r * np.sin(th)
+    p = da.build_normalized_particles(line, xn, 0.0, yn, 0.0, d, nemitt_x, nemitt_y)
+    p.move(_context=dev)
+    n = len(r)
+    ref = _reference(line, None)
+    bg = float(ref._np('beta0')[0] * ref._np('gamma0')[0])
+    sx, sy = math.sqrt(nemitt_x / bg), math.sqrt(nemitt_y / bg)
+    co = find_closed_orbit(line, ref, method='4d')
+    R, _ = compute_one_turn_matrix_finite_differences(line, ref, co, kill_cavity=True)
+    invW4 = get_linear_normal_form(R, only_4d_block=True)[1][:4, :4]
+    # 1. feature turns for every particle, in chunks of particle ids (the
+    # turn-by-turn record holds every field of every particle and turn)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    order = torch.argsort(p._arrays['particle_id'])
+    feat = torch.empty((n, 16), dtype=torch.float32, device=dev)
+    for c0 in range(0, n, feature_chunk):
+        c1 = min(n, c0 + feature_chunk)
+        slots = order[c0:c1]
+        sub = _take(p, slots)
+        mon = ParticlesMonitor(start_at_turn=0, stop_at_turn=feature_turns,
+                               particle_id_range=(c0, c1), auto_to_numpy=False, _context=dev)
+        line.track(sub, num_turns=feature_turns, turn_by_turn_monitor=mon)
+        feat[c0:c1] = features(mon, xn[c0:c1], yn[c0:c1], d[c0:c1], invW4, sx, sy,
+                               feature_turns)
+        for nn, vv in p._arrays.items():
+            if vv.dtype == torch.uint32:
+                vv.view(torch.int32)[slots] = sub._arrays[nn].view(torch.int32)
+            else:
+                vv[slots] = sub._arrays[nn]
+        del mon, sub
+    torch.cuda.synchronize(dev)
+    t_feat = time.perf_counter() - t0
+    surv = np.full(n, -1, dtype=np.int64)
+    st = p._arrays['state'][order].cpu().numpy()
+    at = p._arrays['at_turn'][order].cpu().numpy()
+    surv[st <= 0] = at[st <= 0]
+    alive = np.flatnonzero(st > 0)
+    exact = st <= 0
+    # 2. labels: a random subset of the survivors tracked to num_turns
+    rng = np.random.default_rng(seed)
+    n_train = max(16, int(train_fraction * len(alive))) if len(alive) else 0
+    train = np.sort(rng.choice(alive, size=min(n_train, len(alive)), replace=False))
+
+    def track_rest(ids):
+        if len(ids) == 0:
+            return np.zeros(0, dtype=np.int64)
+        sub = _take(p, order[torch.as_tensor(ids, device=dev)])
+        line.track(sub, num_turns=num_turns - feature_turns)
+        s = sub.sorted_by_id()
+        return np.where(s['state'] > 0, num_turns, s['at_turn'])
+
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    surv[train] = track_rest(train)
+    exact[train] = True
+    torch.cuda.synchronize(dev)
+    t_train_track = time.perf_counter() - t0
+    # 3. fit log(survival) on the matrix cores
+    t0 = time.perf_counter()
+    net = net or MLP(device=dev, seed=seed)
+    fx = feat[torch.as_tensor(train, device=dev)]
+    fy = torch.as_tensor(np.log(surv[train].astype(np.float64)), dtype=torch.float32,
+                         device=dev)
+    hist = net.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(train) // 4)),
+                   seed=seed)
+    pred = np.exp(np.minimum(net.predict(feat).cpu().numpy().astype(np.float64),
+                             math.log(1e12)))
+    torch.cuda.synchronize(dev)
+    t_fit = time.perf_counter() - t0
+    # 4. band around each angle's predicted aperture, tracked in full
+    guess = np.where(exact, surv, np.minimum(pred, num_turns)).astype(np.float64)
+    band_ids = []
+    for a in np.unique(th):
+        for dd in np.unique(d):
+            sel = np.flatnonzero((th == a) & (d == dd))
+            sel = sel[np.argsort(r[sel])]
+            lost = np.flatnonzero(guess[sel] < 0.5 * num_turns)
+            r_pred = r[sel][lost[0]] if len(lost) else r_max
+            inb = sel[(np.abs(r[sel] - r_pred) <= band) & ~exact[sel]]
+            band_ids.append(inb)
+    band_ids = np.sort(np.concatenate(band_ids)) if band_ids else np.zeros(0, np.int64)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    surv[band_ids] = track_rest(band_ids)
+    exact[band_ids] = True
+    torch.cuda.synchronize(dev)
+    t_band = time.perf_counter() - t0
+    # 5. the rest keeps its prediction (survives num_turns when predicted to)
+    rest = ~exact
+    surv[rest] = np.where(pred[rest] >= 0.5 * num_turns, num_turns,
+                          np.minimum(pred[rest], num_turns - 1)).astype(np.int64)
+    return {'r': r, 'theta': th, 'delta': d, 'survival': surv, 'exact': exact,
+            'tracked_full': np.union1d(train, band_ids), 'pred': pred,
+            'da': da.dynamic_aperture(r, th, d, surv, num_turns), 'loss_history': hist,
+            'time': {'features': t_feat, 'train_tracking': t_train_track, 'fit': t_fit,
+                     'band_tracking': t_band, 'total': time.perf_counter() - t_all},
+            'net': net}
