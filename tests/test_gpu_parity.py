@@ -344,17 +344,7 @@ def test_da_scan_survival_turns():
     from xtrack_amd import lattices, da
     print('[da scan]', flush=True)
 
-    def ring():
-        line = lattices.thin_fodo_ring()
-        els = list(line.elements)
-        names = list(line.element_names)
-        for k in range(0, 100, 10):
-            i = names.index(f'q{k}')
-            els.insert(i + 1, xt.Multipole(knl=[0.0, 0.0, 8.0 if k % 20 else -5.0]))
-            names.insert(i + 1, f'sx{k}')
-        out = xt.Line(elements=els, element_names=names)
-        out.particle_ref = line.particle_ref
-        return out
+    ring = lattices.sextupole_fodo_ring
 
     lg, lc = ring(), ring()
     lg.build_tracker(device='cuda:0', compile=_MODE['compile'])

@@ -99,17 +99,8 @@ def test_mlp_fits_a_smooth_function():
 
 
 def _sext_ring():
-    import xtrack_amd as xt
     from xtrack_amd import lattices
-    line = lattices.thin_fodo_ring()
-    els, names = list(line.elements), list(line.element_names)
-    for k in range(0, 100, 10):
-        i = names.index(f'q{k}')
-        els.insert(i + 1, xt.Multipole(knl=[0.0, 0.0, 8.0 if k % 20 else -5.0]))
-        names.insert(i + 1, f'sx{k}')
-    out = xt.Line(elements=els, element_names=names)
-    out.particle_ref = line.particle_ref
-    return out
+    return lattices.sextupole_fodo_ring()
 
 
 def test_surrogate_da_scan_against_plain():
