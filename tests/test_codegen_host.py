@@ -106,7 +106,7 @@ def test_segments_losses_partial_ranges_and_monitors():
     pos = np.full(len(key), -1)
     for e in np.flatnonzero(seg):
         pos[e:e + span[e]] = np.arange(span[e])
-    assert (span > 1).sum() >= 8
+    assert (span > 1).sum() >= 4 and span.sum() >= 100
     p = random_particles(800, seed=1, scale=2e-3)
     for kw in (dict(num_turns=3), dict(num_turns=2, ele_start=5, ele_stop=50),
                dict(ele_start=20, num_elements=70), dict(ele_start=16, ele_stop=27),

@@ -46,7 +46,7 @@ const int kNargs[OP_NUM] = XT_OP_NARGS_INIT;
 // its terms) and that hold no integrator, map or RF op are cheap enough
 // for the per-element dispatch to matter; runs of them form segments.
 constexpr int kSmallCost = 12;
-constexpr size_t kSegMax = 16;           // elements per segment (< 32)
+constexpr size_t kSegMax = 31;           // elements per segment (< 32)
 constexpr int64_t kSegMaxCount = 256;    // distinct segments per line
 constexpr int64_t kSegMaxBody = 2048;    // sum of their lengths
 static_assert(kSegMax <= XT_KEY_SPAN_MASK, "segment length must fit its key field");
