@@ -56,3 +56,23 @@ def test_normalized_particles_follow_the_normal_form():
     assert np.allclose(back[2] / np.sqrt(geps[1]), yn, rtol=1e-9, atol=1e-9)
     assert np.allclose(back[3] / np.sqrt(geps[1]), pyn, rtol=1e-9, atol=1e-9)
     assert np.all(h['delta'] == 1e-3)
+
+
+def test_polar_grid_and_aperture_extraction():
+    """Grid layout, and the aperture as the last amplitude below the first
+    loss at each angle (r_max when nothing is lost, 0 when the first point
+    is lost), on synthetic survival turns."""
+    r, th, d = da.polar_grid(10.0, 10, 4, deltas=(0.0, 1e-3))
+    assert len(r) == 10 * 4 * 2 and np.isclose(r.min(), 1.0) and np.isclose(r.max(), 10.0)
+    assert np.allclose(np.unique(th), (np.arange(4) + 0.5) * np.pi / 8)
+    T = 100
+    surv = np.full(len(r), T)
+    a0, a1 = np.unique(th)[:2]
+    surv[(th == a0) & (d == 0.0) & (r >= 6.0)] = 10                   # lost from 6 sigma
+    surv[(th == a1) & (d == 0.0) & (np.isclose(r, 3.0))] = 50         # isolated loss at 3
+    surv[(th == a0) & (d == 1e-3)] = 1                                 # all lost
+    ap = da.dynamic_aperture(r, th, d, surv, T)
+    assert np.isclose(ap[(float(a0), 0.0)], 5.0)
+    assert np.isclose(ap[(float(a1), 0.0)], 2.0)
+    assert ap[(float(a0), 1e-3)] == 0.0
+    assert np.isclose(ap[(float(np.unique(th)[3]), 0.0)], 10.0)
