@@ -195,6 +195,13 @@ extern "C" {
 #define XTM_FLAG_IGNORE_GLOBAL_APERTURE 3
 #define XTM_FLAG_IGNORE_LOCAL_APERTURE 4
 #define XTM_FLAG_SR_TAPER 5
+/* Line.track(freeze_longitudinal=True) (xtrack/tracker.py:1237-1242): the
+ * reference recompiles with zeta and the energy variables frozen (their
+ * setters become no-ops, particles/particles.py:185-230); here a flag bit
+ * beyond the reference's register: energy kicks are skipped and zeta is
+ * restored after every element, which is the same for every element of
+ * this build (none reads zeta after changing it). */
+#define XTM_FLAG_FREEZE_LONGITUDINAL 16
 
 /* ---- particles: SoA, one pointer per per-particle field --------------------
  * Mirrors ParticlesData (xtrack/particles/particles.py:26-83).  All pointers

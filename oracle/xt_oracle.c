@@ -79,6 +79,12 @@ typedef struct {
 #define GET(f) (part->P->f[part->ipart])
 #define SET(f, v) (part->P->f[part->ipart] = (v))
 #define ADD(f, v) (part->P->f[part->ipart] += (v))
+/* zeta with Line.track(freeze_longitudinal=True): the reference compiles
+   its setters and adders of zeta to no-ops (FREEZE_VAR_zeta,
+   particles/particles.py:185-230) */
+#define ZETA_FROZEN() ((int)((part->track_flags >> XTM_FLAG_FREEZE_LONGITUDINAL) & 1))
+#define ADD_ZETA(v) do { if (!ZETA_FROZEN()) ADD(zeta, (v)); } while (0)
+#define SET_ZETA(v) do { if (!ZETA_FROZEN()) SET(zeta, (v)); } while (0)
 
 static int check_flag(const LP *part, int idx) { return (int)((part->track_flags >> idx) & 1); }
 
@@ -93,6 +99,8 @@ static int check_flag(const LP *part, int idx) { return (int)((part->track_flags
 /* --------------------------------------------------------------------------
  * local_particle_custom_api.h */
 static void update_ptau(LP *part, double new_ptau_value) {     /* :21-32 */
+    /* frozen longitudinal motion: the energy setters are no-ops */
+    if (ZETA_FROZEN()) return;
     double const beta0 = GET(beta0);
     double const ptau = new_ptau_value;
     double const irpp = sqrt(ptau * ptau + 2 * ptau / beta0 + 1);
@@ -201,7 +209,7 @@ static void drift_expanded(LP *part, double length) {
     ADD(x, xp * length);
     ADD(y, yp * length);
     ADD(s, length);
-    ADD(zeta, length * dzeta);
+    ADD_ZETA(length * dzeta);
 }
 
 static void drift_exact(LP *part, double length) {
@@ -213,7 +221,7 @@ static void drift_exact(LP *part, double length) {
     double const dzeta = 1 - rv0v * one_plus_delta * one_over_pz;
     ADD(x, px * one_over_pz * length);
     ADD(y, py * one_over_pz * length);
-    ADD(zeta, dzeta * length);
+    ADD_ZETA(dzeta * length);
     ADD(s, length);
 }
 
@@ -242,7 +250,7 @@ static void polar_drift(LP *part, const double length, const double h) {
     SET(x, new_x);
     SET(px, new_px);
     SET(y, new_y);
-    ADD(zeta, length - delta_ell / rvv);
+    ADD_ZETA(length - delta_ell / rvv);
     ADD(s, s);
 }
 
@@ -329,7 +337,7 @@ static void expanded_dipole_quad(LP *part, const double length, const double k0_
     SET(px, px_);
     SET(y, y_);
     SET(py, py_);
-    ADD(zeta, dzeta);
+    ADD_ZETA(dzeta);
     ADD(s, length);
 }
 
@@ -463,7 +471,7 @@ static void magnet_kick(LP *part, const body_args *a, double kick_weight) {
     dpy += htot * chi * (a->k1_h_correction * length + k1l_mult) * kick_weight * x * y;
     ADD(px, dpx);
     ADD(py, dpy);
-    ADD(zeta, dzeta);
+    ADD_ZETA(dzeta);
 }
 
 /* track_magnet.h:26-285 with XTRACK_MULTIPOLE_NO_SYNRAD */
@@ -1169,7 +1177,7 @@ static void xrotation_single(LP *part, double sin_angle, double cos_angle, doubl
     SET(x, x_hat);
     SET(py, py_hat);
     SET(y, y_hat);
-    SET(zeta, t_hat * beta0);
+    SET_ZETA(t_hat * beta0);
 }
 
 /* track_yrotation.h:11-45 (spin not tracked) */
@@ -1190,7 +1198,7 @@ static void yrotation_single(LP *part, double sin_angle, double cos_angle, doubl
     SET(x, x_hat);
     SET(px, px_hat);
     SET(y, y_hat);
-    SET(zeta, t_hat * beta0);
+    SET_ZETA(t_hat * beta0);
 }
 
 /* The particles a transformation applies to.  The serial CPU and GPU
@@ -1217,7 +1225,7 @@ static void S_SHIFT(const Blk *b, double ds) {
     if (ds != 0.0) {
         FOR_BLK(b, part) {
             drift_exact(part, ds);
-            ADD(zeta, -ds);
+            ADD_ZETA(-ds);
             ADD(s, -ds);
         } END_FOR_BLK
     }

@@ -357,3 +357,17 @@ def test_da_scan_survival_turns():
     surv = rc['survival']
     assert 0 < (surv < 200).sum() < len(surv)
     assert len(np.unique(surv)) > 5
+
+
+def test_freeze_longitudinal_bit_exact():
+    """Line.track(freeze_longitudinal=True): the kernel (energy kicks
+    skipped, zeta restored after every element) against the oracle (the
+    reference's frozen setters)."""
+    print('[freeze longitudinal]', flush=True)
+    p = random_particles(2000, seed=21, scale=4e-3)
+    lg, lc, pg, pc = _pair(lambda: mixed_line(n_cells=3), p)
+    lg.track(pg, num_turns=5, freeze_longitudinal=True)
+    lc.track(pc, num_turns=5, freeze_longitudinal=True)
+    bad = compare_exact(pg, pc)
+    assert not bad, bad
+    assert np.array_equal(pg.sorted_by_id()['zeta'], p.sorted_by_id()['zeta'])

@@ -157,7 +157,7 @@ struct Gen {
             case OP_EDGE_LINEAR: line("xt::op_edge_linear(p" + cs(a, 2) + ");"); break;
             case OP_RF_KICK:
                 line("xt::op_rf_kick(p, m, " + at +
-                     ", c.line_length, c.q0, c.t_sim, c.kill_cavity());");
+                     ", c.line_length, c.q0, c.t_sim, c.kill_energy());");
                 break;
             case OP_LIMIT_RECT:
                 line("xt::op_limit_rect(p" + cs(a, 4) + ", c.ignore_local());");
@@ -174,7 +174,7 @@ struct Gen {
             case OP_RF_MULT:
                 // the descriptor holds program offsets: absolute addressing
                 line("xt::op_rf_mult(p, m, prog + " + std::to_string(pc + 1) +
-                     ", prog, c.q0, c.kill_cavity());");
+                     ", prog, c.q0, c.kill_energy());");
                 break;
             default: return false;
             }

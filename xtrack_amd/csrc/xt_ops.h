@@ -66,7 +66,7 @@ enum { KF_P1 = 1, KF_P2 = 2, KF_P3 = 4, KF_HL = 8, KF_G0 = 16, KF_H1 = 32 };
    every element passed adds one, so it is a per-particle base plus the
    element index (track_particle). */
 /* slots of the energy fields in PState::slow */
-enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_S = 4, XS_N = 5 };
+enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_S = 4, XS_ZETA0 = 5, XS_N = 6 };
 
 template <class R>
 struct PState {
@@ -961,6 +961,9 @@ struct TrackCtx {
     XT_HD bool ignore_local() const { return (flags >> 4) & 1; }   // XS_FLAG_IGNORE_LOCAL_APERTURE
     XT_HD bool ignore_global() const { return (flags >> 3) & 1; }  // XS_FLAG_IGNORE_GLOBAL_APERTURE
     XT_HD bool kill_cavity() const { return (flags >> 2) & 1; }    // XS_FLAG_KILL_CAVITY_KICK
+    XT_HD bool freeze_long() const { return (flags >> 16) & 1; }   // XTM_FLAG_FREEZE_LONGITUDINAL
+    /* energy kicks skipped: killed cavities, or frozen longitudinal motion */
+    XT_HD bool kill_energy() const { return kill_cavity() || freeze_long(); }
 };
 
 /* ---- element program interpreter ------------------------------------------ */
@@ -1015,7 +1018,7 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
             pc += 3;
             break;
         case OP_RF_KICK:
-            op_rf_kick(p, m, a, c.line_length, c.q0, c.t_sim, c.kill_cavity());
+            op_rf_kick(p, m, a, c.line_length, c.q0, c.t_sim, c.kill_energy());
             pc += 7;
             break;
         case OP_LIMIT_RECT:
@@ -1051,7 +1054,7 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
             pc += 3;
             break;
         case OP_RF_MULT:
-            op_rf_mult(p, m, a, prog, c.q0, c.kill_cavity());
+            op_rf_mult(p, m, a, prog, c.q0, c.kill_energy());
             pc += 6;
             break;
         default:
@@ -1191,6 +1194,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
     p.chi = CHI1 ? 1.0 : P.chi[i];
     p.at_turn = P.at_turn[i]; p.state = P.state[i];
     p.put(XS_INV_RVV, 1. / rvv0);
+    p.put(XS_ZETA0, p.zeta);
     refresh_flags(p);
     const PRef m{&P, i};
     const int64_t ele_stop = (int64_t)ele_start + num_ele_track;
@@ -1223,6 +1227,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
                 } else {
                     run(p, m, key, c);
                 }
+                if (c.freeze_long()) p.zeta = p.get(XS_ZETA0);
                 if (p.state <= 0) { e_lost = e + j; break; }
                 e += n;
                 key = next;
@@ -1255,6 +1260,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
             for (int64_t e = ele_stop - 1; e >= ele_start; e--) {
                 if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
                 run(p, m, run.key(e), c);
+                if (c.freeze_long()) p.zeta = p.get(XS_ZETA0);
                 if (p.state <= 0) { e_lost = e; break; }
             }
             if (e_lost >= 0) break;

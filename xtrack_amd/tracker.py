@@ -23,6 +23,9 @@ FLAG_BITS = {
     'XS_FLAG_IGNORE_LOCAL_APERTURE': 4,
     'XS_FLAG_SR_TAPER': 5,
     'XS_FLAG_SR_KICK_SAME_AS_FIRST': 6,
+    # no counterpart in the reference's register: freeze_longitudinal there
+    # recompiles the kernel with zeta and the energy variables frozen
+    'XTM_FLAG_FREEZE_LONGITUDINAL': 16,
 }
 
 
@@ -188,14 +191,18 @@ class Tracker:
         if backtrack is not False and backtrack is not True and backtrack != 'force':
             raise ValueError("backtrack must be True, False or 'force'")
         if freeze_longitudinal:
-            raise NotImplementedError('freeze_longitudinal is not supported by the HIP tracker')
+            self._check_freeze_supported()
         t0 = perf_counter() if time else None
         particles, restore = self._place(particles)
         # tracker.py:1222-1236: backtracking sets XS_FLAG_BACKTRACK for this
         # call (every element of this build has a backtracking map)
         old_bt = self.track_flags.XS_FLAG_BACKTRACK
+        old_fz = self.track_flags.XTM_FLAG_FREEZE_LONGITUDINAL
         if backtrack is not False:
             self.track_flags.XS_FLAG_BACKTRACK = True
+        # tracker.py:1237-1242: zeta and the energy variables stay as they are
+        if freeze_longitudinal:
+            self.track_flags.XTM_FLAG_FREEZE_LONGITUDINAL = True
         try:
             if with_progress:
                 self._track_in_batches(particles, with_progress, ele_start, ele_stop,
@@ -208,9 +215,22 @@ class Tracker:
                 self.backend.synchronize()
         finally:
             self.track_flags.XS_FLAG_BACKTRACK = old_bt
+            self.track_flags.XTM_FLAG_FREEZE_LONGITUDINAL = old_fz
             if restore is not None:
                 restore()
         self.time_last_track = (perf_counter() - t0) if time else None
+
+    def _check_freeze_supported(self):
+        """The kernel freezes zeta by restoring it after each element, which
+        equals the reference's frozen setters unless an element reads zeta
+        after changing it: only an RF element (Cavity, RFMultipole) whose
+        misalignment rotates about x or y or shifts along s does."""
+        for nn, ee in zip(self.line.element_names, self.line.elements):
+            if type(ee).__name__ in ('Cavity', 'RFMultipole') and any(
+                    getattr(ee, a, 0.0) != 0.0 for a in ('rot_x_rad', 'rot_y_rad', 'shift_s')):
+                raise NotImplementedError(
+                    f'freeze_longitudinal with {nn}: an RF element rotated about x or y or '
+                    'shifted along s is not supported')
 
     def _track_in_batches(self, particles, with_progress, ele_start, ele_stop,
                           num_elements, num_turns, turn_by_turn_monitor):
