@@ -24,9 +24,13 @@
  *
  * Not restated (rejected with XTM_ERR_UNSUPPORTED): radiation and spin
  * (compiled out by XTRACK_MULTIPOLE_NO_SYNRAD, the default line config,
- * xtrack/line.py:284), full/dipole-only edge models, the bend-kick-bend
- * model, rbends, solenoids, misalignments, the multi-element monitor and
- * the SR-taper flag.
+ * xtrack/line.py:284), full/dipole-only edge models, rbends, solenoids,
+ * the multi-element monitor and the SR-taper flag.
+ *
+ * The exact dipole maps of the bend-kick-bend model (curved_exact_bend,
+ * straight_exact_bend) are pinned in tests/test_exact_bend.py: design orbit
+ * on the axis, convergence of the Yoshida-4 integrators with many kicks to
+ * them, px - k0 L in a straight frame, symplectic transverse Jacobian.
  */
 #include <math.h>
 #include <stdint.h>
@@ -44,6 +48,8 @@
 #define M_SINH sinh
 #define M_COSH cosh
 #define M_TAN tan
+#define M_ATAN2P atan2
+#define M_ASINP asin
 #define M_EXP exp
 #else
 #include "../xtrack_amd/csrc/xt_math.h"
@@ -52,6 +58,8 @@
 #define M_SINH xt_sinh
 #define M_COSH xt_cosh
 #define M_TAN xt_tan
+#define M_ATAN2P xt_atan2
+#define M_ASINP xt_asin
 #define M_EXP xt_exp
 #endif
 
@@ -342,6 +350,74 @@ static void expanded_dipole_quad(LP *part, const double length, const double k0_
 }
 
 /* track_magnet_drift.h:468-555 (models 0,1,2,3,7,8) */
+/* track_magnet_drift.h:271-346 (caller ensures h != 0) */
+static void curved_exact_bend(LP *part, const double length, const double k0, const double h) {
+    double const k0_chi = k0 * GET(chi);
+    if (fabs(k0_chi) < 1e-8) {
+        polar_drift(part, length, h);
+        return;
+    }
+    const double rvv = GET(rvv);
+    const double x0 = GET(x);
+    const double y0 = GET(y);
+    const double px0 = GET(px);
+    const double py = GET(py);
+    const double s = length;
+    const double one_plus_delta = GET(delta) + 1.0;
+    const double hs = h * s;
+    const double sin_hs = M_SIN(hs);
+    const double cos_hs = M_COS(hs);
+    const double sin_hs_2 = M_SIN(hs / 2);
+    const double pz0 = sqrt(one_plus_delta * one_plus_delta - px0 * px0 - py * py);
+    const double C = pz0 - k0_chi * ((1.0 / h) + x0);
+    const double pxs = px0 * cos_hs + C * sin_hs;
+    const double pzs = sqrt(one_plus_delta * one_plus_delta - pxs * pxs - py * py);
+    const double delta_pz = (px0 - pxs) * (px0 + pxs) / (pz0 + pzs);
+    const double delta_D = -2 * C * (sin_hs_2 * sin_hs_2) - px0 * sin_hs;
+    const double delta_x = (delta_pz - delta_D) / k0_chi;
+    const double delta_px = -2 * px0 * (sin_hs_2 * sin_hs_2) + C * sin_hs;
+    const double N_a = px0 * delta_pz - pz0 * delta_px;
+    const double D_a = pz0 * pzs + px0 * pxs;
+    const double delta_a = M_ATAN2P(N_a, D_a);
+    const double integ = (hs + delta_a) / k0_chi;
+    const double new_y = y0 + py * integ;
+    const double delta_ell = one_plus_delta * integ;
+    ADD(x, delta_x);
+    SET(px, pxs);
+    SET(y, new_y);
+    ADD_ZETA(length - delta_ell / rvv);
+    ADD(s, s);
+}
+
+/* track_magnet_drift.h:348-394 */
+static void straight_exact_bend(LP *part, const double length, const double k0) {
+    double const k0_chi = k0 * GET(chi);
+    if (fabs(k0_chi) < 1e-8) {
+        drift_exact(part, length);
+        return;
+    }
+    const double rvv = GET(rvv);
+    const double x = GET(x);
+    const double y = GET(y);
+    const double px = GET(px);
+    const double py = GET(py);
+    const double s = length;
+    const double one_plus_delta = GET(delta) + 1.0;
+    const double A = 1.0 / sqrt(one_plus_delta * one_plus_delta - py * py);
+    const double pz = sqrt(one_plus_delta * one_plus_delta - px * px - py * py);
+    const double new_px = px - k0_chi * s;
+    const double new_x =
+        x + (sqrt(one_plus_delta * one_plus_delta - new_px * new_px - py * py) - pz) / k0_chi;
+    const double D = M_ASINP(A * px) - M_ASINP(A * new_px);
+    const double new_y = y + (py / k0_chi) * D;
+    const double delta_ell = (one_plus_delta / k0_chi) * D;
+    SET(x, new_x);
+    SET(px, new_px);
+    SET(y, new_y);
+    ADD_ZETA(length - delta_ell / rvv);
+    ADD(s, s);
+}
+
 static void magnet_drift(LP *part, const double length, const double k0, const double k1,
                          const double h, const int64_t drift_model) {
     if (drift_model == -1) return;
@@ -351,6 +427,8 @@ static void magnet_drift(LP *part, const double length, const double k0, const d
     case 1: drift_exact(part, length); break;
     case 2: polar_drift(part, length, h); break;
     case 3: expanded_dipole_quad(part, length, k0, k1, h); break;
+    case 4: curved_exact_bend(part, length, k0, h); break;
+    case 5: straight_exact_bend(part, length, k0); break;
     case 7:
         polar_drift(part, 0.6756035959798289 * length, h);
         SET(px, GET(px) - 1.3512071919596578 * k0 * GET(chi) * length);
@@ -574,7 +652,8 @@ static int configure_tracking_model(int64_t model, double k0, double k1, double 
     else if (model == -1) dm = -1;
     else if (model == 7) dm = h_is_zero ? 1 : 2;
     else if (model == 8) dm = h_is_zero ? 1 : 8;
-    else return XTM_ERR_UNSUPPORTED;  /* 2 bend-kick-bend, -2 solenoid */
+    else if (model == 2) dm = h_is_zero ? 5 : 4;
+    else return XTM_ERR_UNSUPPORTED;  /* -2 solenoid */
     a->ks_drift = 0.0;
     if (dm == -1 || dm == 0 || dm == 1) {
         a->k0_drift = 0.0; a->k1_drift = 0.0; a->h_drift = 0.0;
@@ -588,6 +667,14 @@ static int configure_tracking_model(int64_t model, double k0, double k1, double 
         a->k0_drift = k0; a->k1_drift = k1; a->h_drift = h;
         a->k0_kick = 0.0; a->k1_kick = 0.0; a->h_kick = h;
         a->k0_h_correction = 0.; a->k1_h_correction = k1; a->kick_rot_frame = 0;
+    } else if (dm == 4) {
+        a->k0_drift = k0; a->k1_drift = 0.0; a->h_drift = h;
+        a->k0_kick = 0.0; a->k1_kick = k1; a->h_kick = h;
+        a->k0_h_correction = 0.; a->k1_h_correction = k1; a->kick_rot_frame = 0;
+    } else if (dm == 5) {
+        a->k0_drift = k0; a->k1_drift = 0.0; a->h_drift = 0.0;
+        a->k0_kick = 0.0; a->k1_kick = k1; a->h_kick = 0.0;
+        a->k0_h_correction = 0.; a->k1_h_correction = 0.; a->kick_rot_frame = 0;
     } else { /* 7, 8 */
         a->k0_drift = k0; a->k1_drift = 0.0; a->h_drift = h;
         a->k0_kick = 0.0; a->k1_kick = k1; a->h_kick = h;

@@ -371,3 +371,17 @@ def test_freeze_longitudinal_bit_exact():
     bad = compare_exact(pg, pc)
     assert not bad, bad
     assert np.array_equal(pg.sorted_by_id()['zeta'], p.sorted_by_id()['zeta'])
+
+
+def test_exact_bend_bit_exact():
+    """'bend-kick-bend' bends (exact curved and straight dipole maps, the
+    small-field fallbacks, with kicks of k1 and multipoles), protons and
+    ions with chi != 1."""
+    from test_exact_bend import _bkb_ring, _ions
+    print('[exact bend]', flush=True)
+    for p in (random_particles(3000, seed=22, scale=4e-3), _ions(2000, seed=23)):
+        lg, lc, pg, pc = _pair(_bkb_ring, p)
+        lg.track(pg, num_turns=6)
+        lc.track(pc, num_turns=6)
+        bad = compare_exact(pg, pc)
+        assert not bad, bad

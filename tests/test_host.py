@@ -61,7 +61,6 @@ def test_host_only_line_cannot_track():
 
 
 @pytest.mark.parametrize('el', [
-    xt.Bend(length=1.0, angle=0.1, model='bend-kick-bend'),
     xt.Bend(length=1.0, angle=0.1, edge_entry_model='full'),
     xt.Quadrupole(length=1.0, k1=0.1, edge_entry_active=True),
 ])

@@ -77,7 +77,8 @@ bool configure(int64_t model, double k0, double k1, double h, ModelCfg &c) {
     if (model == 1) model = 3;
     const bool h_is_zero = std::fabs(h) < 1e-8;
     int64_t dm;
-    if (model == 3) dm = h_is_zero ? 1 : 7;
+    if (model == 2) dm = h_is_zero ? 5 : 4;
+    else if (model == 3) dm = h_is_zero ? 1 : 7;
     else if (model == 4) dm = 3;
     else if (model == 5) dm = 1;
     else if (model == 6) dm = 0;
@@ -98,6 +99,14 @@ bool configure(int64_t model, double k0, double k1, double h, ModelCfg &c) {
         c.k0_drift = k0; c.k1_drift = k1; c.h_drift = h;
         c.k0_kick = 0.0; c.k1_kick = 0.0; c.h_kick = h;
         c.k0_h_correction = 0.; c.k1_h_correction = k1; c.kick_rot_frame = 0;
+    } else if (dm == 4) {
+        c.k0_drift = k0; c.k1_drift = 0.0; c.h_drift = h;
+        c.k0_kick = 0.0; c.k1_kick = k1; c.h_kick = h;
+        c.k0_h_correction = 0.; c.k1_h_correction = k1; c.kick_rot_frame = 0;
+    } else if (dm == 5) {
+        c.k0_drift = k0; c.k1_drift = 0.0; c.h_drift = 0.0;
+        c.k0_kick = 0.0; c.k1_kick = k1; c.h_kick = 0.0;
+        c.k0_h_correction = 0.; c.k1_h_correction = 0.; c.kick_rot_frame = 0;
     } else {  // 7, 8
         c.k0_drift = k0; c.k1_drift = 0.0; c.h_drift = h;
         c.k0_kick = 0.0; c.k1_kick = k1; c.h_kick = h;
@@ -155,6 +164,16 @@ struct Builder {
         case 3:
             P.op(OP_DIPQUAD); P.arg(dl); P.arg(c.k0_drift); P.arg(c.k1_drift); P.arg(h);
             return true;
+        case 4: {
+            // track_magnet_drift.h:271-346: hs = h s; sin, cos of hs and of hs/2
+            double ca, sa, t2, ica, s2, c2;
+            polar_consts(h, dl, &ca, &sa, &t2, &ica);
+            const double hs = h * dl;
+            xt_sincos(hs / 2, &s2, &c2);
+            P.op(OP_BEND_CURVED); P.arg(dl); P.arg(c.k0_drift); P.arg(1 / h); P.arg(hs);
+            P.arg(sa); P.arg(ca); P.arg(s2 * s2); P.arg(t2); P.arg(ica);
+            return true; }
+        case 5: P.op(OP_BEND_STRAIGHT); P.arg(dl); P.arg(c.k0_drift); return true;
         case 7: {
             const double a = 0.6756035959798289 * dl;
             const double b = -0.17560359597982889 * dl;

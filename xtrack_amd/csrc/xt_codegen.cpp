@@ -139,6 +139,7 @@ struct Gen {
             const std::string at = addr(pc + 1);
             cost += 1;
             if (op == OP_POLAR || op == OP_DIPQUAD || op == OP_YOSHIDA4_BEND ||
+                op == OP_BEND_CURVED || op == OP_BEND_STRAIGHT ||
                 op == OP_YOSHIDA6_BEND || op == OP_RF_KICK || op == OP_X_ROT || op == OP_Y_ROT ||
                 op == OP_RF_MULT)
                 heavy = true;
@@ -146,6 +147,8 @@ struct Gen {
             case OP_DRIFT_EXP: line("xt::drift_expanded(p" + cs(a, 1) + ");"); break;
             case OP_DRIFT_EXACT: line("xt::drift_exact(p" + cs(a, 1) + ");"); break;
             case OP_POLAR: line("xt::polar_drift(p" + cs(a, 6) + ");"); break;
+            case OP_BEND_CURVED: line("xt::op_bend_curved(p" + cs(a, 9) + ");"); break;
+            case OP_BEND_STRAIGHT: line("xt::op_bend_straight(p" + cs(a, 2) + ");"); break;
             case OP_DIPQUAD: line("xt::dipole_quad(p" + cs(a, 4) + ");"); break;
             case OP_YOSHIDA4_BEND: line("xt::op_yoshida4(p" + cs(a, 14) + ");"); break;
             case OP_YOSHIDA6_BEND: line("xt::op_yoshida6(p, " + at + ");"); break;

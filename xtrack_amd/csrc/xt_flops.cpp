@@ -65,6 +65,8 @@ inline void xsinhcosh(FC v, FC *s, FC *c) {
     *c = FC(b);
 }
 inline FC xsin(FC v) { g.trans += 1; return FC(xt_sin(v.v)); }
+inline FC xatan2(FC y, FC x) { g.trans += 1; return FC(xt_atan2(y.v, x.v)); }
+inline FC xasin(FC v) { g.trans += 1; return FC(xt_asin(v.v)); }
 inline double xval(FC v) { return v.v; }
 }  // namespace xt
 

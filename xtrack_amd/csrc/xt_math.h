@@ -357,4 +357,79 @@ XT_MATH_FN void xt_sinhcosh(double x, double *sh, double *ch) {
 XT_MATH_FN double xt_sinh(double x) { double s, c; xt_sinhcosh(x, &s, &c); return s; }
 XT_MATH_FN double xt_cosh(double x) { double s, c; xt_sinhcosh(x, &s, &c); return c; }
 
+/* ---- atan2 and asin ------------------------------------------------------
+   atan(u) for |u| <= 1: u > tan(pi/8) maps to pi/4 + atan((u - 1)/(u + 1)),
+   one argument halving u -> u / (1 + sqrt(1 + u^2)) (atan u = 2 atan u')
+   leaves |u'| <= 0.2, then the Taylor series sum (-1)^k u'^(2k+1)/(2k+1)
+   with 12 terms (truncation below 2^-60 relative); coefficients are the
+   rationals 1/(2k+1) rounded by the compiler.  atan2 folds the quadrants
+   with pi and pi/2 as two-double constants; asin(z) = atan2(z,
+   sqrt((1 - z)(1 + z))).  Used per particle by the exact bend maps. */
+XT_MATH_FN double xt_atan_small(double u) {      /* |u| <= 0.2 */
+    const double u2 = u * u;
+    double r = -1.0 / 23.0;
+    r = XT_FMA(r, u2, 1.0 / 21.0);
+    r = XT_FMA(r, u2, -1.0 / 19.0);
+    r = XT_FMA(r, u2, 1.0 / 17.0);
+    r = XT_FMA(r, u2, -1.0 / 15.0);
+    r = XT_FMA(r, u2, 1.0 / 13.0);
+    r = XT_FMA(r, u2, -1.0 / 11.0);
+    r = XT_FMA(r, u2, 1.0 / 9.0);
+    r = XT_FMA(r, u2, -1.0 / 7.0);
+    r = XT_FMA(r, u2, 1.0 / 5.0);
+    r = XT_FMA(r, u2, -1.0 / 3.0);
+    return XT_FMA(r * u2, u, u);
+}
+
+/* atan(u) for 0 <= u <= 1, as hi + lo */
+XT_MATH_FN void xt_atan_unit(double u, double *hi, double *lo) {
+    const double pio4_hi = 0x1.921fb54442d18p-1, pio4_lo = 0x1.1a62633145c07p-55;
+    double base_hi = 0.0, base_lo = 0.0;
+    if (u > 0x1.a827999fcef32p-2) {      /* tan(pi/8) */
+        u = (u - 1.0) / (u + 1.0);
+        base_hi = pio4_hi;
+        base_lo = pio4_lo;
+    }
+    const double h = u / (1.0 + XT_SQRT(XT_FMA(u, u, 1.0)));
+    const double a = 2.0 * xt_atan_small(h);
+    *hi = base_hi + a;
+    *lo = (base_hi - *hi) + a + base_lo;
+}
+
+XT_MATH_FN double xt_atan2(double y, double x) {
+    const double pi_hi = 0x1.921fb54442d18p+1, pi_lo = 0x1.1a62633145c07p-53;
+    const double pio2_hi = 0x1.921fb54442d18p+0, pio2_lo = 0x1.1a62633145c07p-54;
+    if (y != y || x != x) return x + y;
+    const double ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
+    const int neg_y = (int)(xt_dbits(y) >> 63), neg_x = (int)(xt_dbits(x) >> 63);
+    double r;
+    if (ay == 0.0) {
+        r = neg_x ? pi_hi : 0.0;
+    } else if (ax == 0.0) {
+        r = pio2_hi;
+    } else if (ax == __builtin_inf() && ay == __builtin_inf()) {
+        r = neg_x ? 3 * (pio2_hi * 0.5) : pio2_hi * 0.5;
+    } else if (ax == __builtin_inf()) {
+        r = neg_x ? pi_hi : 0.0;
+    } else if (ay == __builtin_inf()) {
+        r = pio2_hi;
+    } else {
+        double hi, lo;
+        if (ay <= ax) {
+            xt_atan_unit(ay / ax, &hi, &lo);
+            if (neg_x) { const double t = pi_hi - hi; lo = ((pi_hi - t) - hi) + pi_lo - lo; hi = t; }
+        } else {
+            xt_atan_unit(ax / ay, &hi, &lo);
+            if (neg_x) { const double t = pio2_hi + hi; lo = ((pio2_hi - t) + hi) + pio2_lo + lo; hi = t; }
+            else { const double t = pio2_hi - hi; lo = ((pio2_hi - t) - hi) + pio2_lo - lo; hi = t; }
+        }
+        r = hi + lo;
+    }
+    return neg_y ? -r : r;
+}
+
+XT_MATH_FN double xt_asin(double z) {
+    return xt_atan2(z, XT_SQRT((1.0 - z) * (1.0 + z)));
+}
+
 #endif /* XT_MATH_H */

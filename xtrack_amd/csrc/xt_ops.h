@@ -120,6 +120,8 @@ XT_HD double xsin(double v) { return xt_sin(v); }
 XT_HD double xfabs(double v) { return XT_FABS(v); }
 XT_HD bool xfinite(double v) { return XT_FABS(v) < XT_INF; }
 XT_HD double xval(double v) { return v; }
+XT_HD double xatan2(double y, double x) { return xt_atan2(y, x); }
+XT_HD double xasin(double v) { return xt_asin(v); }
 
 /* a/b given y = RN(1/b): Markstein's correction applied twice,
    q0 = RN(a y), r = a - b q (exact with fma), q' = RN(q + r y).  After the
@@ -754,6 +756,70 @@ XT_HD_COLD RFMOut<R> rf_multipole_kick(const R x, const R y, const R px, const R
     return o;
 }
 
+/* track_magnet_drift.h:271-346: exact map of a sector bend with field k0
+   and curvature h (bend-kick-bend, h != 0); sin(hs), cos(hs) and
+   sin(hs/2)^2 hoisted from the element; a particle with |k0 chi| < 1e-8
+   takes the polar drift (rho = 1/h, t2, ica as for OP_POLAR). */
+template <class R>
+XT_HD void op_bend_curved(PState<R> &p, const double L, const double k0, const double rho,
+                          const double hs, const double sin_hs, const double cos_hs,
+                          const double s2sq, const double t2, const double ica) {
+    const R k0_chi = k0 * p.chi;
+    if (xfabs(xval(k0_chi)) < 1e-8) {
+        polar_drift(p, L, rho, cos_hs, sin_hs, t2, ica);
+        return;
+    }
+    const R rvv = p.rvv();
+    const R x0 = p.x, y0 = p.y, px0 = p.px, py = p.py;
+    const R one_plus_delta = p.delta() + 1.0;
+    const R pz0 = xsqrt(one_plus_delta * one_plus_delta - px0 * px0 - py * py);
+    const R C = pz0 - k0_chi * (rho + x0);
+    const R pxs = px0 * cos_hs + C * sin_hs;
+    const R pzs = xsqrt(one_plus_delta * one_plus_delta - pxs * pxs - py * py);
+    const R delta_pz = (px0 - pxs) * (px0 + pxs) / (pz0 + pzs);
+    const R delta_D = -2 * C * s2sq - px0 * sin_hs;
+    const R delta_x = (delta_pz - delta_D) / k0_chi;
+    const R delta_px = -2 * px0 * s2sq + C * sin_hs;
+    const R N_a = px0 * delta_pz - pz0 * delta_px;
+    const R D_a = pz0 * pzs + px0 * pxs;
+    const R delta_a = xatan2(N_a, D_a);
+    const R integ = (hs + delta_a) / k0_chi;
+    const R new_y = y0 + py * integ;
+    const R delta_ell = one_plus_delta * integ;
+    p.x = p.x + delta_x;
+    p.px = pxs;
+    p.y = new_y;
+    p.zeta = p.zeta + (L - delta_ell / rvv);
+    p.set_s(p.s() + L);
+}
+
+/* track_magnet_drift.h:348-394: exact map of a straight bend (h = 0) with
+   field k0; |k0 chi| < 1e-8 takes the exact drift */
+template <class R>
+XT_HD void op_bend_straight(PState<R> &p, const double L, const double k0) {
+    const R k0_chi = k0 * p.chi;
+    if (xfabs(xval(k0_chi)) < 1e-8) {
+        drift_exact(p, L);
+        return;
+    }
+    const R rvv = p.rvv();
+    const R x = p.x, y = p.y, px = p.px, py = p.py;
+    const R one_plus_delta = p.delta() + 1.0;
+    const R A = 1.0 / xsqrt(one_plus_delta * one_plus_delta - py * py);
+    const R pz = xsqrt(one_plus_delta * one_plus_delta - px * px - py * py);
+    const R new_px = px - k0_chi * L;
+    const R new_x = x + (xsqrt(one_plus_delta * one_plus_delta - new_px * new_px - py * py) - pz) /
+                            k0_chi;
+    const R D = xasin(A * px) - xasin(A * new_px);
+    const R new_y = y + (py / k0_chi) * D;
+    const R delta_ell = (one_plus_delta / k0_chi) * D;
+    p.x = new_x;
+    p.px = new_px;
+    p.y = new_y;
+    p.zeta = p.zeta + (L - delta_ell / rvv);
+    p.set_s(p.s() + L);
+}
+
 /* ---- element ops, shared by the interpreter and generated line code ------- */
 
 /* track_magnet_drift.h:500-509: four polar drifts with the k0 kicks of the
@@ -987,6 +1053,14 @@ XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ p
         case OP_POLAR:
             polar_drift(p, a[0], a[1], a[2], a[3], a[4], a[5]);
             pc += 7;
+            break;
+        case OP_BEND_CURVED:
+            op_bend_curved(p, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+            pc += 10;
+            break;
+        case OP_BEND_STRAIGHT:
+            op_bend_straight(p, a[0], a[1]);
+            pc += 3;
             break;
         case OP_DIPQUAD:
             dipole_quad(p, a[0], a[1], a[2], a[3]);

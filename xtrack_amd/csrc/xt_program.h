@@ -43,12 +43,14 @@ enum xt_op {
     OP_Y_ROT = 18,         /* sin cos tan as YRotation_single_particle takes   */
     OP_S_ROT = 19,         /* sin cos                                          */
     OP_RF_MULT = 20,       /* V freq lag phase desc  (RF multipole kick)       */
-    OP_NUM = 21
+    OP_BEND_CURVED = 21,   /* L k0 1/h hs sin(hs) cos(hs) sin(hs/2)^2 t2 ica   */
+    OP_BEND_STRAIGHT = 22, /* L k0                                              */
+    OP_NUM = 23
 };
 
 /* number of constants following each op word */
 #define XT_NARGS_KICK 6
-#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2, 1, 3, 3, 2, 5}
+#define XT_OP_NARGS_INIT {0, 1, 1, 6, 4, 14, 26, XT_NARGS_KICK, 2, 0, 2, 6, 4, 3, 0, 2, 1, 3, 3, 2, 5, 9, 2}
 
 /* RF multipole descriptor, referenced by OP_RF_MULT through an absolute
  * offset into prog: the order (bits), then for k = 0..order the six
