@@ -391,7 +391,8 @@ int32_t xtm_synchronize(void *stream);
 
 /* Diagnostics: evaluate the library's device elementary functions
  * (fn: 0 sin, 1 cos, 2 sinh, 3 cosh, 4 exp, 5 tan, 6 sqrt, 7 div by x[i+n];
- * the kernel's in-range sequences: 8 reciprocal, 9 sqrt, 10 div by x[i+n])
+ * the kernel's in-range sequences: 8 reciprocal, 9 sqrt, 10 div by x[i+n];
+ * 11 atan2(x[i], x[i+n]), 12 asin)
  * on n device doubles.  No counterpart in the reference; used to check that
  * the device and host builds of the functions agree bit for bit and that the
  * in-range sequences return the IEEE results. */

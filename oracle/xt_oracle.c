@@ -1780,6 +1780,8 @@ int32_t xto_math_eval(int32_t fn, const double *x, double *y, int64_t n) {
         case 5: y[i] = M_TAN(v); break;
         case 6: y[i] = sqrt(v); break;
         case 7: y[i] = v / x[i + n]; break;
+        case 11: y[i] = M_ATAN2P(v, x[i + n]); break;
+        case 12: y[i] = M_ASINP(v); break;
         default: return XTM_ERR_ARG;
         }
     }

@@ -38,7 +38,7 @@ def oracle_lib(math='xm'):
 def oracle_math(fn, x, math='xm'):
     lib = oracle_lib(math)
     x = np.ascontiguousarray(x, dtype=np.float64)
-    n = len(x) if fn != 7 else len(x) // 2
+    n = len(x) // 2 if fn in (7, 11) else len(x)
     y = np.empty(n)
     assert lib.xto_math_eval(fn, x.ctypes.data, y.ctypes.data, n) == 0
     return y

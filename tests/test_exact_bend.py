@@ -143,3 +143,24 @@ def test_xm_close_to_libm():
     assert np.array_equal(a['state'], b['state']) and alive.sum() > 100
     for ff in ('x', 'px', 'y', 'py', 'zeta'):
         assert np.max(np.abs(a[ff][alive] - b[ff][alive])) < 1e-12, ff
+
+
+def test_atan2_asin_within_4_ulp():
+    """The project's atan2 and asin (used by the exact maps) against the
+    platform libm: at most 4 ulp apart, the same special values."""
+    from oracle_backend import oracle_math
+    rng = np.random.default_rng(17)
+    n = 100000
+    a = rng.normal(size=n) * 10 ** rng.uniform(-30, 30, n)
+    b = rng.normal(size=n) * 10 ** rng.uniform(-30, 30, n)
+    sp = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-300])
+    a[:64], b[:64] = np.repeat(sp, 8), np.tile(sp, 8)
+    z = np.concatenate([rng.uniform(-1, 1, n), 1 - 10 ** rng.uniform(-16, -1, 1000),
+                        [0.0, -0.0, 1.0, -1.0, 1.0 + 2e-16, np.nan]])
+    for got, want in ((oracle_math(11, np.concatenate([a, b]), 'xm'), np.arctan2(a, b)),
+                      (oracle_math(12, z, 'xm'), np.arcsin(z))):
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert np.array_equal(np.signbit(got[ok]), np.signbit(want[ok]))
+        ulp = np.abs(got[ok] - want[ok]) / np.spacing(np.abs(want[ok]))
+        assert ulp.max() <= 4.0, ulp.max()

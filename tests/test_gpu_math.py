@@ -9,7 +9,8 @@ from oracle_backend import oracle_math
 
 pytestmark = pytest.mark.gpu
 
-FN = {'sin': 0, 'cos': 1, 'sinh': 2, 'cosh': 3, 'exp': 4, 'tan': 5, 'sqrt': 6, 'div': 7}
+FN = {'sin': 0, 'cos': 1, 'sinh': 2, 'cosh': 3, 'exp': 4, 'tan': 5, 'sqrt': 6, 'div': 7,
+      'atan2': 11, 'asin': 12}
 
 
 def _inputs(fn, rng, n):
@@ -31,11 +32,19 @@ def test_device_math_bit_exact(fn):
     lib = _native.load()
     rng = np.random.default_rng(FN[fn])
     n = 200000
-    if fn == 'div':
+    if fn in ('div', 'atan2'):
         a = rng.normal(size=n) * 10 ** rng.uniform(-30, 30, n)
         b = rng.normal(size=n) * 10 ** rng.uniform(-30, 30, n)
+        if fn == 'atan2':
+            sp = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-300])
+            a[:64], b[:64] = np.repeat(sp, 8), np.tile(sp, 8)
         x = np.concatenate([a, b])
         m = n
+    elif fn == 'asin':
+        x = np.concatenate([rng.uniform(-1, 1, n), rng.uniform(-1e-8, 1e-8, n // 4),
+                            1 - 10 ** rng.uniform(-16, -1, n // 4),
+                            [0.0, -0.0, 1.0, -1.0, 0.5, 1.0 + 2e-16, np.nan]])
+        m = len(x)
     else:
         x = _inputs(fn, rng, n)
         m = len(x)

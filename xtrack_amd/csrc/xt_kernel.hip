@@ -133,7 +133,9 @@ __global__ void math_kernel(int32_t fn, const double *__restrict__ x, double *__
     case 7: r = v / x[i + n]; break;
     case 8: r = xt::rcp_ir(v); break;
     case 9: r = xt::sqrt_ir(v); break;
-    default: r = xt::div_ir(v, x[i + n]); break;
+    case 10: r = xt::div_ir(v, x[i + n]); break;
+    case 11: r = xt_atan2(v, x[i + n]); break;
+    default: r = xt_asin(v); break;
     }
     y[i] = r;
 }
@@ -802,7 +804,7 @@ int32_t xtm_synchronize(void *stream) {
 }
 
 int32_t xtm_math_eval(int32_t fn, const double *x, double *y, int64_t n, void *stream) {
-    if (fn < 0 || fn > 10 || n < 0 || (n > 0 && (!x || !y)))
+    if (fn < 0 || fn > 12 || n < 0 || (n > 0 && (!x || !y)))
         return fail(XTM_ERR_ARG, "xtm_math_eval: invalid arguments");
     if (n == 0) return XTM_OK;
     const int64_t nb = (n + BLOCK - 1) / BLOCK;
