@@ -156,26 +156,189 @@ def measure_traffic(args):
     return res
 
 
-def cpu_baseline(line_fn, tw, n, turns):
-    """Oracle (platform libm) on `n` particles x `turns` turns with all host
-    threads the process may use."""
+def _oracle_backend(math, threads):
     sys.path.insert(0, os.path.join(ROOT, 'tests'))
     from oracle_backend import OracleBackend
-    from xtrack_amd import optics
-    threads = len(os.sched_getaffinity(0))
+    return OracleBackend(math=math, num_threads=threads)
+
+
+def cpu_leg(line_fn, parts, turns, threads, what):
+    """Oracle (platform libm: the reference's CPU arithmetic and its OpenMP
+    chunking) tracking `parts` for `turns` turns on `threads` host threads;
+    particle-turns/s."""
     line = line_fn()
-    line.build_tracker(backend=OracleBackend(math='libm', num_threads=threads))
-    p = optics.matched_gaussian(line, line.particle_ref, n, NEMITT, NEMITT, SIGMA_Z,
-                                SIGMA_DELTA, seed=2, tw=tw)
+    line.build_tracker(backend=_oracle_backend('libm', threads))
+    p = parts.copy()
     t0 = time.perf_counter()
     line.track(p, num_turns=turns)
     dt = time.perf_counter() - t0
-    done = int(np.sum(p.to_numpy()['at_turn']))
+    done = int(np.sum(p.to_numpy()['at_turn']) - np.sum(parts.to_numpy()['at_turn']))
     return {'value': done / dt, 'unit': 'particle-turns/s', 'cores': threads,
             'kind': 'port',
-            'sample': f'{n} particles x {turns} turns of the same ring and beam '
-                      f'({done} particle-turns in {dt:.1f} s), CPU restatement of the '
-                      f'reference tracker with platform libm, OpenMP {threads} threads'}
+            'sample': f'{what}: {p._capacity} particles x {turns} turns ({done} particle-turns '
+                      f'in {dt:.1f} s), CPU restatement of the reference tracker with '
+                      f'platform libm, OpenMP {threads} thread(s)'}
+
+
+def cpu_baseline(make_line, beam, thin_line, thin_beam, args):
+    """The reported CPU baseline: the collider ring on a bounded sample of
+    its beam with every host thread, plus a one-thread leg of the same ring
+    and the thin FODO workload at its full size (10,000 particles x 100
+    turns) with every thread and with one."""
+    threads = len(os.sched_getaffinity(0))
+    main_leg = cpu_leg(make_line, _subset(beam, np.arange(args.cpu_sample_particles)),
+                       args.cpu_sample_turns, threads, 'collider ring, matched beam')
+    main_leg['legs'] = {
+        'collider_1_thread': cpu_leg(make_line, _subset(beam, np.arange(512)), 4, 1,
+                                     'collider ring, matched beam'),
+        'thin_fodo_full': cpu_leg(thin_line, thin_beam, 100, threads,
+                                  'thin FODO ring (10,000 particles, 100 turns)'),
+        'thin_fodo_full_1_thread': cpu_leg(thin_line, thin_beam, 100, 1,
+                                           'thin FODO ring (10,000 particles, 100 turns)'),
+    }
+    return main_leg
+
+
+def _subset(parts, idx):
+    """Particles holding entries `idx` of the host set `parts` (same
+    particle ids)."""
+    import xtrack_amd as xt
+    h = parts.to_numpy()
+    out = xt.Particles(p0c=float(h['p0c'][0]), mass0=parts.mass0, q0=parts.q0,
+                       x=np.zeros(len(idx)))
+    out._set_all({nn: np.ascontiguousarray(vv[idx]) for nn, vv in h.items()})
+    return out
+
+
+COORDS = ('x', 'px', 'y', 'py', 'zeta', 'delta')
+
+
+def _compare(got, ref):
+    """Loss records and coordinate differences of two sorted particle
+    dicts: losses_equal (state, at_turn, at_element identical), the number
+    lost, identical bits of every compared field, and per coordinate
+    max |got - ref| / max |ref| over the particles alive in both."""
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    from helpers import COMPARED
+    same = all(np.array_equal(got[ff], ref[ff]) for ff in ('state', 'at_turn', 'at_element'))
+    bits = True
+    for ff in COMPARED:
+        a, b = got[ff], ref[ff]
+        if a.dtype.kind == 'f':
+            eq = ((a == b) & (np.signbit(a) == np.signbit(b))) | (np.isnan(a) & np.isnan(b))
+        else:
+            eq = a == b
+        bits = bits and bool(eq.all())
+    alive = (got['state'] > 0) & (ref['state'] > 0)
+    err = {}
+    for ff in COORDS:
+        den = float(np.max(np.abs(ref[ff][alive]))) if alive.any() else 0.0
+        num = float(np.max(np.abs(got[ff][alive] - ref[ff][alive]))) if alive.any() else 0.0
+        err[ff] = num / den if den > 0 else num
+    return {'losses_equal': bool(same), 'lost': int((ref['state'] <= 0).sum()),
+            'bit_identical': bits, 'max_rel_err': max(err.values()), 'rel_err': err}
+
+
+def _oracle_track(line_fn, parts, turns, math, threads, snapshots=()):
+    """Oracle track of `parts`; sorted copies after each turn count in
+    `snapshots` and at the end."""
+    line = line_fn()
+    line.build_tracker(backend=_oracle_backend(math, threads))
+    p = parts.copy()
+    out, done = {}, 0
+    for t in sorted(set(snapshots) | {turns}):
+        line.track(p, num_turns=t - done)
+        done = t
+        out[t] = p.sorted_by_id()
+    return out
+
+
+def collider_parity_sample(beam, n_core=48, n_edge=16, seed=3):
+    """Particles of the collider beam plus particles at the edge of the
+    horizontal aperture (amplitude 7.5-8.7 mm at the start, where the
+    dispersion is 2.5 m), which synchrotron motion loses at scattered
+    turns."""
+    import xtrack_amd as xt
+    core = _subset(beam, np.arange(n_core)).to_numpy()
+    rng = np.random.default_rng(seed)
+    edge = dict(x=rng.uniform(0.0075, 0.0087, n_edge) * rng.choice([-1, 1], n_edge),
+                px=rng.normal(0, 3e-6, n_edge), y=rng.normal(0, 1e-3, n_edge),
+                py=rng.normal(0, 3e-6, n_edge), zeta=rng.normal(0, 0.1, n_edge),
+                delta=rng.normal(0, 2e-4, n_edge))
+    kw = {ff: np.concatenate([core[ff][:n_core], edge[ff]]) for ff in edge}
+    return xt.Particles(p0c=float(core['p0c'][0]), mass0=beam.mass0, q0=beam.q0, **kw)
+
+
+def parity_and_cpu_legs(args, make_line, parts_gpu, init, turns_done):
+    """GPU against the oracle on the same inputs, after the timed region:
+    (1) a sample of the timed beam after the bench's turns; (2) the thin
+    FODO workload at its full size (10,000 matched particles) after 1 and
+    1000 turns; (3) 64 particles of the collider workload (48 of the
+    matched beam, 16 at the aperture edge) after 1 and 1000 turns, tracked
+    on a second stream while the CPU runs the oracle and the baseline legs.
+    Each against the oracle with the project's math library (the kernel's
+    arithmetic: bits must be identical) and with the platform libm (the
+    reference's arithmetic: losses identical, coordinate error stated)."""
+    import torch
+    from xtrack_amd import lattices, optics
+    threads = len(os.sched_getaffinity(0))
+    res = {'tolerance': {'after_1_turn': 1e-12, 'after_1000_turns': 1e-9},
+           'error_definition': 'per coordinate max |GPU - oracle| / max |oracle| over '
+                               'particles alive in both; losses_equal: state, at_turn and '
+                               'at_element identical'}
+    print('bench.py: parity legs', file=sys.stderr, flush=True)
+    # (1) sample of the timed beam
+    rng = np.random.default_rng(77)
+    idx = np.sort(rng.choice(init._capacity, 256, replace=False))
+    g_beam = _subset(parts_gpu, idx).sorted_by_id()
+    samp = _subset(init, idx)
+    # (2) thin FODO workload on the GPU
+    thin = lattices.thin_fodo_ring()
+    thin.build_tracker(device='cuda:0')
+    thin_beam = optics.matched_gaussian(thin, thin.particle_ref, 10_000, 2.5e-6, 2.5e-6, 0.2,
+                                        1e-3, seed=0)
+    pg = thin_beam.copy(_context='cuda:0')
+    thin.track(pg, num_turns=1)
+    g_thin = {1: pg.sorted_by_id()}
+    thin.track(pg, num_turns=999)
+    g_thin[1000] = pg.sorted_by_id()
+    # (3) collider sample: 1 turn, then 999 turns on a second stream
+    csamp = collider_parity_sample(init)
+    cline = make_line()
+    cline.build_tracker(device='cuda:0')
+    cg = csamp.copy(_context='cuda:0')
+    s2 = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s2):
+        cline.track(cg, num_turns=1)
+        s2.synchronize()
+        g_col = {1: cg.sorted_by_id()}
+        cline.track(cg, num_turns=999)
+    # CPU work while the GPU tracks
+    o_beam = {m: _oracle_track(make_line, samp, turns_done, m, threads)[turns_done]
+              for m in ('xm', 'libm')}
+    o_thin = {m: _oracle_track(lattices.thin_fodo_ring, thin_beam, 1000, m, threads, (1,))
+              for m in ('xm', 'libm')}
+    o_col = {m: _oracle_track(make_line, csamp, 1000, m, threads, (1,)) for m in ('xm', 'libm')}
+    cpu = cpu_baseline(make_line, init, lattices.thin_fodo_ring, thin_beam, args)
+    s2.synchronize()
+    g_col[1000] = cg.sorted_by_id()
+
+    def both(g, o):
+        return {'vs_oracle_project_math': _compare(g, o['xm']),
+                'vs_oracle_libm': _compare(g, o['libm'])}
+
+    res['collider_beam_sample'] = dict(particles=len(idx), turns=turns_done,
+                                       **both(g_beam, o_beam))
+    res['thin_fodo_10k'] = {'particles': 10_000,
+                            'after_1_turn': both(g_thin[1], {m: o_thin[m][1] for m in o_thin}),
+                            'after_1000_turns': both(g_thin[1000],
+                                                     {m: o_thin[m][1000] for m in o_thin})}
+    res['collider_64'] = {'particles': 64, 'of_which_at_aperture_edge': 16,
+                          'after_1_turn': both(g_col[1], {m: o_col[m][1] for m in o_col}),
+                          'after_1000_turns': both(g_col[1000],
+                                                   {m: o_col[m][1000] for m in o_col})}
+    return res, cpu
 
 
 def main_batch(args):
@@ -251,8 +414,9 @@ def main_batch(args):
                                        int(sum((p.to_numpy()['state'] > 0).sum()
                                                for p in parts))],
                    'host_setup_s': round(setup_s, 1), 'wall_s': round(elapsed, 2)},
-        'roofline': {'bound': 'mfma', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
-                                                      '78.6 TFLOP/s, vector and matrix)',
+        'roofline': {'bound': 'fp64-valu', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
+                                                           '78.6 TFLOP/s; no matrix '
+                                                           'instructions issued)',
                      'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None},
     }
@@ -342,6 +506,7 @@ def main():
     chroma = optics.chromaticity(line, ref)
     parts = optics.matched_gaussian(line, ref, args.particles, NEMITT, NEMITT, SIGMA_Z,
                                     SIGMA_DELTA, seed=1, tw=tw)
+    init = None if args.no_cpu_baseline else parts.copy()
     parts.move(_context='cuda:0')
 
     # algorithmic operations per particle-turn (kernel element code, counted)
@@ -409,8 +574,9 @@ def main():
                    'particles_alive_end': n_alive,
                    'tunes': [round(float(tw['qx']), 4), round(float(tw['qy']), 4)],
                    'chromaticity': [round(float(chroma[0]), 3), round(float(chroma[1]), 3)]},
-        'roofline': {'bound': 'mfma', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
-                                                      '78.6 TFLOP/s, vector and matrix)',
+        'roofline': {'bound': 'fp64-valu', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
+                                                           '78.6 TFLOP/s; no matrix '
+                                                           'instructions issued)',
                      'achieved': achieved, 'peak': PEAK_FP64_TFLOPS,
                      'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP64_TFLOPS,
                      'traffic': traffic_bytes,
@@ -423,8 +589,8 @@ def main():
                      'kernel_ms': [round(float(x), 3) for x in kern_ms]},
     }
     if not args.no_cpu_baseline:
-        out['cpu_baseline'] = cpu_baseline(make_line, tw, args.cpu_sample_particles,
-                                           args.cpu_sample_turns)
+        out['parity'], out['cpu_baseline'] = parity_and_cpu_legs(
+            args, make_line, parts, init, (args.warmup + args.steps) * tps)
     print(json.dumps(out), flush=True)
 
 

@@ -164,3 +164,63 @@ def test_atan2_asin_within_4_ulp():
         assert np.array_equal(np.signbit(got[ok]), np.signbit(want[ok]))
         ulp = np.abs(got[ok] - want[ok]) / np.spacing(np.abs(want[ok]))
         assert ulp.max() <= 4.0, ulp.max()
+
+
+@pytest.mark.parametrize('kw', [dict(k1=0.002), dict(k1=-0.003, knl=[0, 0, 0.01]),
+                                dict(k0=0.05 / 3.0 * 1.2, k1=0.001, k2=0.05)])
+def test_curved_map_with_gradient_converges(kw):
+    """Bend-kick-bend with a gradient (and multipoles): the exact dipole map
+    between kicks of k1 (and knl) with k0_h_correction off.  With the kick
+    count growing it converges to the Yoshida-4 rot-kick-rot integrator of
+    the same magnet; a wrong sign or a dropped gradient term in the kick
+    split does not converge."""
+    p = _beam(seed=12)
+
+    def bend(**extra):
+        return xt.Bend(length=3.0, angle=0.05, **kw, **extra)
+
+    exact = _track([bend(model='bend-kick-bend', num_multipole_kicks=32)], p,
+                   OracleBackend(math='libm'))
+    errs = []
+    for n in (32, 64, 128):
+        ref = _track([bend(model='rot-kick-rot', integrator='yoshida4',
+                           num_multipole_kicks=n)], p, OracleBackend(math='libm'))
+        errs.append(max(np.max(np.abs(exact[ff] - ref[ff])) for ff in
+                        ('x', 'px', 'y', 'py', 'zeta')))
+    # fourth-order convergence of the integrator onto the exact-map result
+    # (factor 16 per halving of the step; >= 8 required)
+    assert errs[2] < 1e-11 and errs[1] < errs[0] / 8 and errs[2] < errs[1] / 8, errs
+
+
+@pytest.mark.parametrize('math', ['xm', 'libm'])
+def test_exact_maps_backtrack_round_trip(math):
+    """Forward through the bend-kick-bend ring (curved and straight exact
+    maps, with and without gradient) and back: the initial coordinates
+    within 5e-13 (track_magnet.h:414-442 reverses the kick sequence and
+    negates the lengths)."""
+    p = random_particles(300, seed=21, scale=2e-3)
+    line = _bkb_ring()
+    line.build_tracker(backend=OracleBackend(math=math))
+    q = p.copy()
+    line.track(q)
+    line.track(q, backtrack=True)
+    a, b = q.sorted_by_id(), p.sorted_by_id()
+    alive = a['state'] > 0
+    assert alive.sum() > 250
+    for ff in ('x', 'px', 'y', 'py', 'zeta', 'delta'):
+        assert np.max(np.abs(a[ff][alive] - b[ff][alive])) < 5e-13, ff
+
+
+def test_exact_maps_backtrack_host_kernel_bit_exact():
+    """The kernel's backtracking programs of the exact maps, compiled for the
+    host, give the oracle's bits."""
+    p = random_particles(400, seed=22, scale=3e-3)
+    lh = _bkb_ring()
+    lh.build_tracker(backend=HostKernelBackend())
+    lc = _bkb_ring()
+    lc.build_tracker(backend=OracleBackend())
+    ph, pc = p.copy(), p.copy()
+    for ll, pp in ((lh, ph), (lc, pc)):
+        ll.track(pp, num_turns=3)
+        ll.track(pp, num_turns=2, backtrack=True)
+    assert not compare_exact(ph, pc)

@@ -216,6 +216,20 @@ def test_backtracking_bit_exact(kw):
                                   getattr(lc.record_last_track, ff), equal_nan=True), ff
 
 
+def test_backtracking_exact_bends_bit_exact():
+    """Backtracking through the exact bend-kick-bend maps (curved and
+    straight frames, with gradient and multipole kicks)."""
+    from test_exact_bend import _bkb_ring
+    print('[backtrack exact bends]', flush=True)
+    p = random_particles(1500, seed=18, scale=4e-3)
+    lg, lc, pg, pc = _pair(_bkb_ring, p)
+    for ll, pp in ((lg, pg), (lc, pc)):
+        ll.track(pp, num_turns=3)
+        ll.track(pp, num_turns=2, backtrack=True)
+    bad = compare_exact(pg, pc)
+    assert not bad, bad
+
+
 def test_sps_lattice_bit_exact():
     """The SPS line of the reference's test data (JSON: Drift, DipoleEdge,
     Multipole, apertures, one shifted LimitRect, cavities) with losses."""
