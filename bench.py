@@ -50,6 +50,12 @@ FLOP_PER_CALL = 20     # weight of one elementary-function call
 # in place (ptau beta0 charge_ratio p0c ax ay), counted by the traffic runs
 STATE_BYTES_PER_PARTICLE = (13 + 12) * 8
 COPY_MIB = 256
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (each stage of a run prints one)."""
+    print(f'bench.py [{time.perf_counter() - _T0:7.1f} s]: {msg}', file=sys.stderr, flush=True)
 
 
 def parse():
@@ -62,6 +68,13 @@ def parse():
     ap.add_argument('--cpu-sample-particles', type=int, default=8192)
     ap.add_argument('--cpu-sample-turns', type=int, default=24)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--arithmetic', choices=('exact', 'fast'), default='exact',
+                    help='arithmetic of the headline run: exact (IEEE quotients in the '
+                         "reference's order) or fast (re-associated polar-drift path length)")
+    ap.add_argument('--no-variants', action='store_true',
+                    help='skip the timed run of the other arithmetic')
+    ap.add_argument('--no-batch', action='store_true',
+                    help='skip the batched-variants leg of the default run')
     ap.add_argument('--traffic', choices=('measure', 'none'), default='measure',
                     help='measure HBM traffic with two rocprofv3 --pmc child runs')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -140,8 +153,9 @@ def measure_traffic(args):
                    '--warmup', '0', '--turns-per-step', str(args.turns_per_step),
                    '--particles', str(args.particles), '--no-cpu-baseline',
                    '--traffic', 'none', '--copy-calibration', '--compile', args.compile,
+                   '--arithmetic', args.arithmetic, '--no-variants', '--no-batch',
                    '--waves-per-eu', str(args.waves_per_eu)]
-            print(f'bench.py: traffic pass {counter}', file=sys.stderr, flush=True)
+            log(f'traffic pass {counter}')
             with open(os.path.join(tmp, counter + '.log'), 'w') as log:
                 subprocess.run(cmd, stdout=log, stderr=subprocess.STDOUT, timeout=420,
                                check=True)
@@ -149,8 +163,8 @@ def measure_traffic(args):
             scale = copy_bytes / (kib_copy * 1024.0)
             res[counter] = {'kib_track': kib_track, 'kib_copy': kib_copy, 'scale': scale,
                             'bytes': kib_track * 1024.0 * scale}
-            print(f'bench.py: {counter} track {kib_track:.0f} KiB, copy {kib_copy:.0f} KiB '
-                  f'(scale {scale:.4f})', file=sys.stderr, flush=True)
+            log(f'{counter} track {kib_track:.0f} KiB, copy {kib_copy:.0f} KiB '
+                f'(scale {scale:.4f})')
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return res
@@ -274,11 +288,12 @@ def parity_and_cpu_legs(args, make_line, parts_gpu, init, turns_done):
     (1) a sample of the timed beam after the bench's turns; (2) the thin
     FODO workload at its full size (10,000 matched particles) after 1 and
     1000 turns; (3) 64 particles of the collider workload (48 of the
-    matched beam, 16 at the aperture edge) after 1 and 1000 turns, tracked
-    on a second stream while the CPU runs the oracle and the baseline legs.
-    Each against the oracle with the project's math library (the kernel's
-    arithmetic: bits must be identical) and with the platform libm (the
-    reference's arithmetic: losses identical, coordinate error stated)."""
+    matched beam, 16 at the aperture edge) after 1 and 1000 turns in each
+    arithmetic, tracked on two more streams while the CPU runs the oracle
+    and the baseline legs.  Each against the oracle with the project's math
+    library (the kernel's elementary functions: in exact arithmetic the bits
+    must be identical) and with the platform libm (the reference's
+    arithmetic: losses identical, coordinate error stated)."""
     import torch
     from xtrack_amd import lattices, optics
     threads = len(os.sched_getaffinity(0))
@@ -286,7 +301,7 @@ def parity_and_cpu_legs(args, make_line, parts_gpu, init, turns_done):
            'error_definition': 'per coordinate max |GPU - oracle| / max |oracle| over '
                                'particles alive in both; losses_equal: state, at_turn and '
                                'at_element identical'}
-    print('bench.py: parity legs', file=sys.stderr, flush=True)
+    log(f'parity legs ({threads} host threads)')
     # (1) sample of the timed beam
     rng = np.random.default_rng(77)
     idx = np.sort(rng.choice(init._capacity, 256, replace=False))
@@ -302,52 +317,67 @@ def parity_and_cpu_legs(args, make_line, parts_gpu, init, turns_done):
     g_thin = {1: pg.sorted_by_id()}
     thin.track(pg, num_turns=999)
     g_thin[1000] = pg.sorted_by_id()
-    # (3) collider sample: 1 turn, then 999 turns on a second stream
+    log('thin FODO workload tracked on the GPU (1000 turns)')
+    # (3) collider sample in both arithmetics: 1 turn, then 999 turns on
+    # their own streams
     csamp = collider_parity_sample(init)
-    cline = make_line()
-    cline.build_tracker(device='cuda:0')
-    cg = csamp.copy(_context='cuda:0')
-    s2 = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    with torch.cuda.stream(s2):
-        cline.track(cg, num_turns=1)
-        s2.synchronize()
-        g_col = {1: cg.sorted_by_id()}
-        cline.track(cg, num_turns=999)
+    g_col, runs = {}, []
+    for arith in ('exact', 'fast'):
+        cline = make_line()
+        cline.build_tracker(device='cuda:0', arithmetic=arith)
+        cg = csamp.copy(_context='cuda:0')
+        st = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(st):
+            cline.track(cg, num_turns=1)
+            st.synchronize()
+            g_col[arith] = {1: cg.sorted_by_id()}
+            cline.track(cg, num_turns=999)
+        runs.append((arith, cline, cg, st))
+    log('collider sample: 1 turn done, 999 turns queued')
     # CPU work while the GPU tracks
     o_beam = {m: _oracle_track(make_line, samp, turns_done, m, threads)[turns_done]
               for m in ('xm', 'libm')}
+    log('oracle: beam sample')
     o_thin = {m: _oracle_track(lattices.thin_fodo_ring, thin_beam, 1000, m, threads, (1,))
               for m in ('xm', 'libm')}
+    log('oracle: thin FODO workload')
     o_col = {m: _oracle_track(make_line, csamp, 1000, m, threads, (1,)) for m in ('xm', 'libm')}
+    log('oracle: collider sample')
     cpu = cpu_baseline(make_line, init, lattices.thin_fodo_ring, thin_beam, args)
-    s2.synchronize()
-    g_col[1000] = cg.sorted_by_id()
+    log('CPU baseline legs')
+    for arith, cline, cg, st in runs:
+        st.synchronize()
+        with torch.cuda.stream(st):
+            g_col[arith][1000] = cg.sorted_by_id()
+    log('collider sample: 1000 turns done on the GPU')
 
     def both(g, o):
         return {'vs_oracle_project_math': _compare(g, o['xm']),
                 'vs_oracle_libm': _compare(g, o['libm'])}
 
     res['collider_beam_sample'] = dict(particles=len(idx), turns=turns_done,
-                                       **both(g_beam, o_beam))
+                                       arithmetic=args.arithmetic, **both(g_beam, o_beam))
     res['thin_fodo_10k'] = {'particles': 10_000,
                             'after_1_turn': both(g_thin[1], {m: o_thin[m][1] for m in o_thin}),
                             'after_1000_turns': both(g_thin[1000],
                                                      {m: o_thin[m][1000] for m in o_thin})}
-    res['collider_64'] = {'particles': 64, 'of_which_at_aperture_edge': 16,
-                          'after_1_turn': both(g_col[1], {m: o_col[m][1] for m in o_col}),
-                          'after_1000_turns': both(g_col[1000],
-                                                   {m: o_col[m][1000] for m in o_col})}
+    for arith in ('exact', 'fast'):
+        res[f'collider_64_{arith}'] = {
+            'particles': 64, 'of_which_at_aperture_edge': 16, 'arithmetic': arith,
+            'after_1_turn': both(g_col[arith][1], {m: o_col[m][1] for m in o_col}),
+            'after_1000_turns': both(g_col[arith][1000], {m: o_col[m][1000] for m in o_col})}
     return res, cpu
 
 
-def main_batch(args):
+def batch_leg(args, steps):
     """Batched lattice variants: `--batch-variants` variants of the ring
     (seeds 0.., cells per arc uniform in 8..40, quadrupole and sextupole
     strengths scattered by N(0, 1e-4) relative), particle counts log-uniform
     in [1000, --batch-max-particles], all tracked in one launch per step of
     `--turns-per-step` turns.  Beams: Gaussian, the same small emittance for
-    every variant (not matched to each variant's optics)."""
+    every variant (not matched to each variant's optics).  Returns the
+    measurement as a dict."""
     import torch
     import xtrack_amd as xt
     from xtrack_amd import lattices, _native
@@ -365,8 +395,10 @@ def main_batch(args):
                                   y=r.normal(0, 3e-4, n), py=r.normal(0, 3e-6, n),
                                   zeta=r.normal(0, SIGMA_Z, n),
                                   delta=r.normal(0, SIGMA_DELTA, n)))
+    log(f'batch: {nv} lines built ({time.perf_counter() - t0:.1f} s)')
     bt = xt.BatchTracker(lines, device='cuda:0', compile=args.batch_compile)
     setup_s = time.perf_counter() - t0
+    log(f'batch: tracker ready ({setup_s:.1f} s)')
     # operations per particle-turn of each variant (same counting as the ring)
     lib = _native.load()
     flop = np.zeros(nv)
@@ -377,6 +409,7 @@ def main_batch(args):
                                              xt.PROTON_MASS_EV, 1.0, ops.ctypes.data))
         flop[k] = ops[5] + FLOP_PER_CALL * ops[4]
     cfg = bt.tune(parts, turns=1)
+    log(f'batch: autotuned {cfg}')
     tps = args.turns_per_step
     for _ in range(args.warmup):
         bt.track(parts, num_turns=tps, turns_per_launch=tps, use_graph=args.batch_graph)
@@ -384,7 +417,7 @@ def main_batch(args):
     alive0 = np.array([int((p.to_numpy()['state'] > 0).sum()) for p in parts])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    info = bt.track(parts, num_turns=tps * args.steps, turns_per_launch=tps,
+    info = bt.track(parts, num_turns=tps * steps, turns_per_launch=tps,
                     use_graph=args.batch_graph)
     elapsed = time.perf_counter() - t1
     after = np.array([int(p.to_numpy()['at_turn'].sum()) for p in parts])
@@ -392,33 +425,42 @@ def main_batch(args):
     kern_s = info['kernel_ms'] / 1e3
     achieved = float(np.sum(flop * (after - before))) / kern_s / 1e12
     nele = [len(ln.elements) for ln in lines]
+    log(f'batch: {steps} steps, {info["kernel_ms"] / steps:.1f} ms per step')
+    return {
+        'value': pturns / kern_s, 'unit': 'particle-turns/s',
+        'ms_per_step': info['kernel_ms'] / steps, 'steps': steps,
+        'workload': f'{nv} variants of the collider ring (cells per arc 8..40, '
+                    'quad/sextupole errors 1e-4), one launch per step',
+        'variants': nv, 'elements_min_max': [min(nele), max(nele)],
+        'particles_total': int(counts.sum()),
+        'particles_min_max': [int(counts.min()), int(counts.max())],
+        'turns_per_step': tps,
+        'kernel': ('compiled batch code (element structures)' if bt.is_compiled
+                   else 'element-program interpreter (batch)'),
+        'graph': bool(info['graph']), 'autotuned': list(cfg),
+        'tune_ms': {f'{k[0]}x{k[1]}': round(v, 2) for k, v in bt.tune_timings.items()},
+        'alive_start_end': [int(alive0.sum()),
+                            int(sum((p.to_numpy()['state'] > 0).sum() for p in parts))],
+        'host_setup_s': round(setup_s, 1), 'wall_s': round(elapsed, 2),
+        'roofline': {'bound': 'fp64-valu', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP64_TFLOPS},
+    }
+
+
+def main_batch(args):
+    """`--workload batch`: the batched-variants measurement as the line."""
+    b = batch_leg(args, args.steps)
     out = {
         'metric': 'particle-turns/s (FP64) on 1 MI355X, batched lattice variants',
-        'value': pturns / kern_s, 'unit': 'particle-turns/s', 'n_gpus': 1,
-        'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': info['kernel_ms'] / args.steps, 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'value': b['value'], 'unit': 'particle-turns/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': b['ms_per_step'],
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic (seeded Gaussian beams, synthetic lattice variants)',
-        'config': {'workload': f'{nv} variants of the collider ring (cells per arc 8..40, '
-                               'quad/sextupole errors 1e-4), one launch per step',
-                   'variants': nv, 'elements_min_max': [min(nele), max(nele)],
-                   'particles_total': int(counts.sum()),
-                   'particles_min_max': [int(counts.min()), int(counts.max())],
-                   'turns_per_step': tps,
-                   'kernel': ('compiled batch code (element structures)' if bt.is_compiled
-                              else 'element-program interpreter (batch)'),
-                   'graph': bool(info['graph']), 'autotuned': list(cfg),
-                   'tune_ms': {f'{k[0]}x{k[1]}': round(v, 2)
-                               for k, v in bt.tune_timings.items()},
-                   'alive_start_end': [int(alive0.sum()),
-                                       int(sum((p.to_numpy()['state'] > 0).sum()
-                                               for p in parts))],
-                   'host_setup_s': round(setup_s, 1), 'wall_s': round(elapsed, 2)},
-        'roofline': {'bound': 'fp64-valu', 'compute_pipe': 'FP64 VALU (MI355X FP64 peak '
-                                                           '78.6 TFLOP/s; no matrix '
-                                                           'instructions issued)',
-                     'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None},
+        'config': {k: v for k, v in b.items()
+                   if k not in ('value', 'unit', 'ms_per_step', 'steps', 'roofline')},
+        'roofline': dict(b['roofline'], compute_pipe='FP64 VALU (MI355X FP64 peak 78.6 '
+                                                     'TFLOP/s; no matrix instructions '
+                                                     'issued)', traffic=None),
     }
     print(json.dumps(out), flush=True)
 
@@ -465,6 +507,30 @@ def main_da(args):
     print(json.dumps(out), flush=True)
 
 
+def timed_steps(line, parts, steps, warmup, tps):
+    """`warmup` untimed then `steps` timed launches of `tps` turns each on
+    the current stream: (completed particle-turns, wall seconds, per-launch
+    kernel ms from HIP events on the launch stream)."""
+    import torch
+    for _ in range(warmup):
+        line.track(parts, num_turns=tps)
+    torch.cuda.synchronize()
+    at_turn0 = parts.at_turn.sum().item()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        line.track(parts, num_turns=tps)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    pturns = parts.at_turn.sum().item() - at_turn0
+    return pturns, elapsed, [a.elapsed_time(b) for a, b in ev]
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -483,7 +549,7 @@ def main():
         try:
             traffic = measure_traffic(args)
         except Exception as exc:      # counters unavailable: report null traffic
-            print(f'bench.py: traffic not measured ({exc})', file=sys.stderr, flush=True)
+            log(f'traffic not measured ({exc})')
 
     import torch
     from xtrack_amd import lattices, optics, _native
@@ -499,15 +565,16 @@ def main():
     line = make_line()
     tracker = line.build_tracker(device='cuda:0',
                                  compile={'auto': 'auto', 'on': True, 'off': False}[args.compile],
-                                 waves_per_eu=args.waves_per_eu)
+                                 waves_per_eu=args.waves_per_eu, arithmetic=args.arithmetic)
     compiled = tracker.backend.is_compiled(tracker._handle)
     ref = line.particle_ref
     tw = optics.twiss(line, ref)
     chroma = optics.chromaticity(line, ref)
     parts = optics.matched_gaussian(line, ref, args.particles, NEMITT, NEMITT, SIGMA_Z,
                                     SIGMA_DELTA, seed=1, tw=tw)
-    init = None if args.no_cpu_baseline else parts.copy()
+    init = parts.copy()
     parts.move(_context='cuda:0')
+    log(f'beam of {args.particles} particles on the GPU, line compiled: {compiled}')
 
     # algorithmic operations per particle-turn (kernel element code, counted)
     lib = _native.load()
@@ -526,25 +593,9 @@ def main():
         del src, dst
 
     tps = args.turns_per_step
-    for _ in range(args.warmup):
-        line.track(parts, num_turns=tps)
-    torch.cuda.synchronize()
-    at_turn0 = parts.at_turn.sum().item()
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        line.track(parts, num_turns=tps)
-        ev[k][1].record(stream)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    at_turn1 = parts.at_turn.sum().item()
-    pturns = at_turn1 - at_turn0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    pturns, elapsed, kern_ms = timed_steps(line, parts, args.steps, args.warmup, tps)
     n_alive = int((parts.state > 0).sum().item())
+    log(f'timed run ({args.arithmetic}): {np.mean(kern_ms):.2f} ms per step')
 
     value = pturns / elapsed
     # per launch: all particles alive at the start of the step x tps turns
@@ -588,9 +639,27 @@ def main():
                      'kernel_ms_mean': float(np.mean(kern_ms)),
                      'kernel_ms': [round(float(x), 3) for x in kern_ms]},
     }
+    out['config']['arithmetic'] = args.arithmetic
+    if not args.no_variants:
+        other = 'fast' if args.arithmetic == 'exact' else 'exact'
+        vline = make_line()
+        vline.build_tracker(device='cuda:0', waves_per_eu=args.waves_per_eu, arithmetic=other)
+        vparts = init.copy(_context='cuda:0')
+        vpt, vel, vms = timed_steps(vline, vparts, args.steps, args.warmup, tps)
+        vach = flop_pt * (vpt / args.steps) / (float(np.mean(vms)) / 1e3) / 1e12
+        out['variants'] = {other: {
+            'value': vpt / vel, 'ms_per_step': vel / args.steps * 1e3,
+            'kernel_ms_mean': float(np.mean(vms)),
+            'roofline_frac': vach / PEAK_FP64_TFLOPS, 'achieved': vach,
+            'same_beam_and_steps': True}}
+        log(f'timed run ({other}): {np.mean(vms):.2f} ms per step')
+        del vparts
+    if not args.no_batch:
+        out['batch'] = batch_leg(args, steps=2)
     if not args.no_cpu_baseline:
         out['parity'], out['cpu_baseline'] = parity_and_cpu_legs(
             args, make_line, parts, init, (args.warmup + args.steps) * tps)
+    log('done')
     print(json.dumps(out), flush=True)
 
 

@@ -281,6 +281,23 @@ int64_t xtm_line_num_records(const xtm_line *line);
  * compilation or loading fails (message in xtm_last_error). */
 int32_t xtm_line_specialize(xtm_line *line, int32_t waves_per_eu);
 
+/* Options of xtm_line_specialize_ex (bit mask). */
+#define XTM_SPEC_FAST_ARITH 1u
+
+/* xtm_line_specialize with options (no counterpart in the reference, which
+ * compiles one arithmetic).  XTM_SPEC_FAST_ARITH: the polar drifts of sector
+ * bends drop their divisions -- the path-length quotient is re-associated
+ * onto the factor the map already forms, and the two remaining reciprocals
+ * (of the square root and of ca - sa px/pz) come from the hardware
+ * estimates with two Newton steps -- so coordinates differ from the IEEE
+ * evaluation by a few ulp per element (about 1e-15 relative after one turn
+ * of a 28k-element ring); losses are decided by the same aperture tests.
+ * xtm_line_specialize(line, w) is xtm_line_specialize_ex(line, w, 0). */
+int32_t xtm_line_specialize_ex(xtm_line *line, int32_t waves_per_eu, uint32_t options);
+
+/* The options of the line's compiled code, or -1 when it has none. */
+int32_t xtm_line_specialized_options(const xtm_line *line);
+
 /* 1 when the line tracks with generated code, else 0. */
 int32_t xtm_line_is_specialized(const xtm_line *line);
 
@@ -293,6 +310,9 @@ int32_t xtm_line_is_specialized(const xtm_line *line);
  * host-only handles; the CPU tests compile the source for the host. */
 int32_t xtm_line_generate_source(const xtm_line *line, int32_t waves_per_eu, char *buf,
                                  int64_t cap, int64_t *len, int32_t *rec_of_elem);
+int32_t xtm_line_generate_source_ex(const xtm_line *line, int32_t waves_per_eu,
+                                    uint32_t options, char *buf, int64_t cap, int64_t *len,
+                                    int32_t *rec_of_elem);
 int64_t xtm_line_program_size(const xtm_line *line);
 int32_t xtm_line_copy_program(const xtm_line *line, double *out);
 int32_t xtm_line_copy_entry(const xtm_line *line, int64_t *out);

@@ -127,10 +127,11 @@ class HostKernelBackend(OracleBackend):
 _gen_cache = {}
 
 
-def generated_host_line(types, offsets, params, line_length):
+def generated_host_line(types, offsets, params, line_length, options=0):
     """(lib, rec, prog): the library's generated line code for this element
-    table, compiled for the host with g++, plus the record index per element
-    and the element program it refers to."""
+    table (xtm_line_generate_source_ex options), compiled for the host with
+    g++, plus the record index per element and the element program it
+    refers to."""
     import hashlib
     import tempfile
     lib = _native.load()
@@ -141,11 +142,11 @@ def generated_host_line(types, offsets, params, line_length):
     try:
         n = ctypes.c_int64()
         rec = np.zeros(max(len(types), 1), dtype=np.int32)
-        _native.check(lib.xtm_line_generate_source(h, 0, None, 0, ctypes.byref(n),
-                                                   rec.ctypes.data))
+        _native.check(lib.xtm_line_generate_source_ex(h, 0, options, None, 0, ctypes.byref(n),
+                                                      rec.ctypes.data))
         buf = ctypes.create_string_buffer(n.value + 1)
-        _native.check(lib.xtm_line_generate_source(h, 0, buf, n.value + 1, ctypes.byref(n),
-                                                   None))
+        _native.check(lib.xtm_line_generate_source_ex(h, 0, options, buf, n.value + 1,
+                                                      ctypes.byref(n), None))
         prog = np.zeros(lib.xtm_line_program_size(h), dtype=np.float64)
         _native.check(lib.xtm_line_copy_program(h, prog.ctypes.data))
         entry = np.zeros(max(len(types), 1), dtype=np.int64)
@@ -176,10 +177,18 @@ def generated_host_line(types, offsets, params, line_length):
 
 
 class GeneratedHostBackend(OracleBackend):
-    """Same protocol, running the library's generated line code on the host."""
+    """Same protocol, running the library's generated line code on the host
+    (options: xtm_line_generate_source_ex bits; with the fast arithmetic the
+    host build takes exact reciprocals where the device refines hardware
+    estimates)."""
+
+    def __init__(self, options=0):
+        super().__init__()
+        self.options = options
 
     def prepare_line(self, types, offsets, params, line_length):
-        glib, rec, prog, _, entry = generated_host_line(types, offsets, params, line_length)
+        glib, rec, prog, _, entry = generated_host_line(types, offsets, params, line_length,
+                                                        self.options)
         return glib, rec, entry, prog, line_length
 
     def track_line(self, handle, particles, num_turns, ele_start, num_ele_track,

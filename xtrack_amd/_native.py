@@ -78,6 +78,7 @@ def make_monitor_struct(monitor):
 
 XTM_ERR_ARG = -1
 XTM_ERR_UNSUPPORTED = -2
+XTM_SPEC_FAST_ARITH = 1   # xtm_line_specialize_ex option bit
 XTM_ERR_HIP = -3
 
 
@@ -127,6 +128,13 @@ def load():
     lib.xtm_line_specialize.argtypes = [c.c_void_p, c.c_int32]
     lib.xtm_line_is_specialized.restype = c.c_int32
     lib.xtm_line_is_specialized.argtypes = [c.c_void_p]
+    lib.xtm_line_specialize_ex.restype = c.c_int32
+    lib.xtm_line_specialize_ex.argtypes = [c.c_void_p, c.c_int32, c.c_uint32]
+    lib.xtm_line_specialized_options.restype = c.c_int32
+    lib.xtm_line_specialized_options.argtypes = [c.c_void_p]
+    lib.xtm_line_generate_source_ex.restype = c.c_int32
+    lib.xtm_line_generate_source_ex.argtypes = [c.c_void_p, c.c_int32, c.c_uint32, c.c_char_p,
+                                                c.c_int64, c.POINTER(c.c_int64), c.c_void_p]
     lib.xtm_line_generate_source.restype = c.c_int32
     lib.xtm_line_generate_source.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64,
                                              c.POINTER(c.c_int64), c.c_void_p]

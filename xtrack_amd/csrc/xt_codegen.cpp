@@ -190,7 +190,7 @@ struct Gen {
 }  // namespace
 
 int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
-                            int32_t waves_per_eu, std::string &src,
+                            int32_t waves_per_eu, uint32_t options, std::string &src,
                             std::vector<int32_t> &rec_of_elem, std::string &err) {
     std::map<int64_t, int32_t> index;
     for (int64_t e : entry) index.emplace(e, 0);
@@ -206,6 +206,7 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
 
     std::string s;
     s += "// generated line code: element records as straight-line calls\n";
+    if (options & XTM_SPEC_FAST_ARITH) s += "#define XTM_ARITH_FAST 1\n";
     s += "#if defined(__HIPCC__) && !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
     s += "#include \"xt_ops.h\"\n\n";
     s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";

@@ -26,9 +26,11 @@ static_assert(XT_CODEGEN_MAX_RECORDS <= XT_KEY_REC_MASK + 1, "record index must 
 // source includes xt_ops.h.  `rec_of_elem[e]` receives the key of element
 // e (its record index and, at the start of a segment of cheap records, the
 // segment fields above).  `waves_per_eu` (> 0) sets the kernel's occupancy target.
+// `options`: XTM_SPEC_* bits (XTM_SPEC_FAST_ARITH compiles the element maps
+// with XTM_ARITH_FAST defined).
 // Returns an XTM_* status and sets `err` on failure.
 int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
-                            int32_t waves_per_eu, std::string &src,
+                            int32_t waves_per_eu, uint32_t options, std::string &src,
                             std::vector<int32_t> &rec_of_elem, std::string &err);
 
 // Source of a translation unit that tracks a batch of lines in one launch:

@@ -172,6 +172,7 @@ struct xtm_line {
     int32_t *d_rec = nullptr;
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;
+    uint32_t spec_options = 0;
 };
 
 struct xtm_batch {
@@ -351,7 +352,13 @@ int32_t xtm_line_destroy(xtm_line *L) {
 }
 
 int32_t xtm_line_specialize(xtm_line *L, int32_t waves_per_eu) {
+    return xtm_line_specialize_ex(L, waves_per_eu, 0);
+}
+
+int32_t xtm_line_specialize_ex(xtm_line *L, int32_t waves_per_eu, uint32_t options) {
     if (!L) return fail(XTM_ERR_ARG, "xtm_line_specialize: null line");
+    if (options & ~XTM_SPEC_FAST_ARITH)
+        return fail(XTM_ERR_ARG, "xtm_line_specialize: unknown option bits");
     if (L->device < 0)
         return fail(XTM_ERR_ARG, "xtm_line_specialize: line was created without a device");
     if (waves_per_eu < 0 || waves_per_eu > 8)
@@ -359,7 +366,7 @@ int32_t xtm_line_specialize(xtm_line *L, int32_t waves_per_eu) {
     if (waves_per_eu == 0) waves_per_eu = XTM_COMPILED_WAVES_PER_EU;
     std::string src, err;
     std::vector<int32_t> rec;
-    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, src, rec, err);
+    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, options, src, rec, err);
     if (st != XTM_OK) return fail(st, "xtm_line_specialize: " + err);
     std::vector<char> code;
     st = rtc_compile(src, code);
@@ -392,17 +399,30 @@ int32_t xtm_line_specialize(xtm_line *L, int32_t waves_per_eu) {
     L->d_rec = d_rec;
     L->module = mod;
     L->kernel = fn;
+    L->spec_options = options;
     return XTM_OK;
 }
 
 int32_t xtm_line_is_specialized(const xtm_line *L) { return (L && L->kernel) ? 1 : 0; }
 
+int32_t xtm_line_specialized_options(const xtm_line *L) {
+    return (L && L->kernel) ? (int32_t)L->spec_options : -1;
+}
+
 int32_t xtm_line_generate_source(const xtm_line *L, int32_t waves_per_eu, char *buf,
                                  int64_t cap, int64_t *len, int32_t *rec_of_elem) {
+    return xtm_line_generate_source_ex(L, waves_per_eu, 0, buf, cap, len, rec_of_elem);
+}
+
+int32_t xtm_line_generate_source_ex(const xtm_line *L, int32_t waves_per_eu, uint32_t options,
+                                    char *buf, int64_t cap, int64_t *len,
+                                    int32_t *rec_of_elem) {
     if (!L || !len) return fail(XTM_ERR_ARG, "xtm_line_generate_source: null argument");
+    if (options & ~XTM_SPEC_FAST_ARITH)
+        return fail(XTM_ERR_ARG, "xtm_line_generate_source: unknown option bits");
     std::string src, err;
     std::vector<int32_t> rec;
-    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, src, rec, err);
+    int st = xt_generate_line_source(L->entry, L->prog, waves_per_eu, options, src, rec, err);
     if (st != XTM_OK) return fail(st, "xtm_line_generate_source: " + err);
     *len = (int64_t)src.size();
     if (buf && cap > 0) {

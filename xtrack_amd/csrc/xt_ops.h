@@ -238,6 +238,30 @@ XT_HD void drift_exact(PState<R> &p, const double length) {
     p.set_s(p.s() + length);
 }
 
+/* Fast arithmetic (line code compiled with XTM_ARITH_FAST): the polar
+   drift's path-length term re-associated.  The reference evaluates
+
+       delta_ell = (1+delta) (x+rho) sa / ca / pz / (1 - px sa / ca / pz)
+
+   with five divisions; the same quantity is (1+delta) * pst with
+   pst = (x+rho) sa (1/pz) (1/(ca - sa px/pz)), the factor the map already
+   forms for y.  DzReassoc<double> selects that form: one product instead
+   of four correctly rounded quotients, then the reference's quotient by
+   rvv.  x, px, y and s keep the reference's operations and therefore its
+   bits; zeta differs by the rounding of the two forms (a few ulp of
+   delta_ell per drift, no bias).  The operation-counting scalar keeps the
+   reference form, so the algorithmic count does not change. */
+template <class R>
+struct DzReassoc {
+    static constexpr bool value = false;
+};
+#if defined(XTM_ARITH_FAST)
+template <>
+struct DzReassoc<double> {
+    static constexpr bool value = true;
+};
+#endif
+
 /* Path-length term of the polar drift, the reference's literal divisions:
    length - (n1/ca/pz/(1 - n3/ca/pz))/rvv.  Out of line so that the
    reciprocal-division path below cannot be if-converted into computing
@@ -283,6 +307,7 @@ struct PolarOut { double x, px, y, dz; };
 /* track_magnet_drift.h:45-87 with IEEE operations throughout: the fallback
    of polar_drift_ir, out of line by default (scalar arguments, four results
    in VGPRs). */
+template <bool RE>
 #ifdef XTM_POLAR_FALLBACK_INLINE
 XT_HD PolarOut polar_plain(
 #else
@@ -303,6 +328,10 @@ XT_HD_COLD PolarOut polar_plain(
     o.x = (x + rho * (t2 + sapxt)) * _ptt;
     o.px = ca * px + sa * pz;
     o.y = y + pst * py;
+    if (RE) {
+        o.dz = length - (one_plus_delta * pst) / rvv;
+        return o;
+    }
     const double delta_ell = one_plus_delta * xr * sa / ca / pz / (1 - px * sa / ca / pz);
     o.dz = length - delta_ell / rvv;
     return o;
@@ -334,15 +363,22 @@ __device__ __forceinline__ void polar_drift_ir(PState<double> &p, const double l
     double new_x = (x + rho * (t2 + sapxt)) * _ptt;
     double new_px = ca * px + sa * pz;
     double new_y = y + pst * py;
-    const double n1 = one_plus_delta * xr * sa;
-    const double n3 = px * sa;
-    const double q2 = div_rcp(div_rcp(n1, ca, ica), pz, _pz);
-    const double q4 = div_rcp(div_rcp(n3, ca, ica), pz, _pz);
-    const double den = 1 - q4;
-    const double delta_ell = div_ir(q2, den);
-    double dz = length - div_rcp(delta_ell, rvv, p.inv_rvv());
+    constexpr bool RE = DzReassoc<double>::value;
+    double dz;
+    if constexpr (RE) {
+        dz = length - div_rcp(one_plus_delta * pst, rvv, p.inv_rvv());
+    } else {
+        const double n1 = one_plus_delta * xr * sa;
+        const double n3 = px * sa;
+        const double q2 = div_rcp(div_rcp(n1, ca, ica), pz, _pz);
+        const double q4 = div_rcp(div_rcp(n3, ca, ica), pz, _pz);
+        const double den = 1 - q4;
+        const double delta_ell = div_ir(q2, den);
+        dz = length - div_rcp(delta_ell, rvv, p.inv_rvv());
+    }
     if (!polar_fast_ok(p, ica)) {
-        const PolarOut o = polar_plain(x, px, y, py, one_plus_delta, rvv, length, rho, ca, sa, t2);
+        const PolarOut o =
+            polar_plain<RE>(x, px, y, py, one_plus_delta, rvv, length, rho, ca, sa, t2);
         new_x = o.x;
         new_px = o.px;
         new_y = o.y;
@@ -382,9 +418,14 @@ XT_HD void polar_drift(PState<R> &p, const double length, const double rho, cons
     const R new_px = ca * px + sa * pz;
     const R new_y = y + pst * py;
     /* delta_ell = opd*(x+rho)*sa/ca/pz/(1 - px*sa/ca/pz) */
-    const R n1 = one_plus_delta * xr * sa;
-    const R n3 = px * sa;
-    const R dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv(), polar_fast_ok(p, ica), length);
+    R dz;
+    if constexpr (DzReassoc<R>::value) {
+        dz = length - (one_plus_delta * pst) / rvv;
+    } else {
+        const R n1 = one_plus_delta * xr * sa;
+        const R n3 = px * sa;
+        dz = polar_dz(n1, n3, ca, ica, pz, _pz, rvv, p.inv_rvv(), polar_fast_ok(p, ica), length);
+    }
     p.x = new_x;
     p.px = new_px;
     p.y = new_y;

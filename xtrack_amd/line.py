@@ -45,7 +45,7 @@ class Line:
         return tot
 
     def build_tracker(self, _context=None, backend=None, device=None, compile='auto',
-                      waves_per_eu=0):
+                      waves_per_eu=0, arithmetic='exact'):
         """Build the device element table.  ``_context``/``device`` pick the
         HIP device (default ``cuda:0``); ``backend`` substitutes another
         implementation of the two-method backend protocol.
@@ -56,10 +56,17 @@ class Line:
         element records and otherwise uses the element-program interpreter,
         True requires compiled code, False always interprets.
         ``waves_per_eu``: occupancy target of the compiled kernel (0: the
-        library default)."""
+        library default).
+        ``arithmetic``: 'exact' (default) evaluates every quotient and square
+        root as the correctly rounded IEEE operation, in the reference's
+        order, so coordinates match the reference's arithmetic bit for bit
+        given the same elementary functions; 'fast' (compiled code only)
+        re-associates the sector-bend polar drift onto reciprocals with a few
+        ulp of error per element.  Particles with chi != 1 and backtracking
+        always run the exact interpreter."""
         dev = device or (_context if isinstance(_context, str) else None) or 'cuda:0'
         self.tracker = Tracker(self, backend=backend, device=dev, compile=compile,
-                               waves_per_eu=waves_per_eu)
+                               waves_per_eu=waves_per_eu, arithmetic=arithmetic)
         return self.tracker
 
     def discard_tracker(self):

@@ -60,18 +60,23 @@ class HIPBackend:
     """Device element table + kernel launches through libxtrack_amd.so."""
     space = 'device'
 
-    def __init__(self, device='cuda:0', compile='auto', waves_per_eu=0):
+    def __init__(self, device='cuda:0', compile='auto', waves_per_eu=0, arithmetic='exact'):
         import torch
+        if compile not in ('auto', True, False):
+            raise ValueError("compile must be 'auto', True or False")
+        if arithmetic not in ('exact', 'fast'):
+            raise ValueError("arithmetic must be 'exact' or 'fast'")
+        if arithmetic == 'fast' and compile is False:
+            raise ValueError("arithmetic='fast' needs compiled line code (compile=True/'auto')")
         self.lib = _native.load()
         if self.lib.xtm_device_count() < 1:
             raise _native.NativeError('no HIP device visible to libxtrack_amd.so')
         self.device = torch.device(device)
         if self.device.index is None:
             self.device = torch.device('cuda:0')
-        if compile not in ('auto', True, False):
-            raise ValueError("compile must be 'auto', True or False")
         self.compile = compile
         self.waves_per_eu = int(waves_per_eu)
+        self.arithmetic = arithmetic
 
     def prepare_line(self, types, offsets, params, line_length):
         handle = ctypes.c_void_p()
@@ -81,7 +86,8 @@ class HIPBackend:
             ctypes.c_int32(self.device.index), ctypes.byref(handle)))
         lh = _LineHandle(self.lib, handle)
         if self.compile is not False:
-            st = self.lib.xtm_line_specialize(handle, self.waves_per_eu)
+            opts = _native.XTM_SPEC_FAST_ARITH if self.arithmetic == 'fast' else 0
+            st = self.lib.xtm_line_specialize_ex(handle, self.waves_per_eu, opts)
             # too many distinct records: 'auto' keeps the interpreter
             if st != 0 and not (st == _native.XTM_ERR_UNSUPPORTED and self.compile == 'auto'):
                 _native.check(st)
@@ -131,10 +137,12 @@ class _LineHandle:
 
 
 class Tracker:
-    def __init__(self, line, backend=None, device='cuda:0', compile='auto', waves_per_eu=0):
+    def __init__(self, line, backend=None, device='cuda:0', compile='auto', waves_per_eu=0,
+                 arithmetic='exact'):
         self.line = line
         self.backend = (backend if backend is not None
-                        else HIPBackend(device, compile=compile, waves_per_eu=waves_per_eu))
+                        else HIPBackend(device, compile=compile, waves_per_eu=waves_per_eu,
+                                        arithmetic=arithmetic))
         self.track_flags = TrackFlags()
         self.num_elements = len(line.elements)
         self.line_length = line.get_length()
