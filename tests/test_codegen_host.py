@@ -126,3 +126,36 @@ def test_segments_losses_partial_ranges_and_monitors():
     for ff in ('x', 'px', 'y', 'py', 'zeta', 'at_element', 'state'):
         assert np.array_equal(getattr(lg.record_last_track, ff),
                               getattr(lc.record_last_track, ff), equal_nan=True), ff
+
+
+def _hybrid_line():
+    """segment_line with 80 thick quadrupoles of individual strengths
+    inserted: more than 64 distinct records, so single-use records get
+    structure functions (constants loaded) and the keys carry program
+    offsets, while the repeated cheap records keep literal segments whose
+    literal-kick fallback addresses each element's own descriptor."""
+    base = segment_line(n_blocks=20, aperture=0.012)
+    els = list(base.elements)
+    rng = np.random.default_rng(5)
+    out = []
+    for k, ee in enumerate(els):
+        out.append(ee)
+        if k % 7 == 3 and len(out) < 10_000:
+            out.append(xt.Quadrupole(length=0.3, k1=0.02 * (1 + 0.1 * rng.standard_normal()),
+                                     knl=[0, 0, 0.01 * rng.standard_normal()]))
+    return xt.Line(elements=out)
+
+
+def test_hybrid_line_code_structures_and_segments():
+    line = _hybrid_line()
+    types, offsets, params = build_raw_table(line.elements)
+    _, rec, _, src, entry = generated_host_line(types, offsets, params, line.get_length())
+    assert len(np.unique(entry)) > 64
+    assert 'struct Key { int32_t r; int64_t pc; }' in src
+    assert 'xt::cptr(prog) + ent[' in src and 'seg_0' in src
+    assert src.count('XT_GEN_FN void rec_') < 30
+    for kw in (dict(num_turns=3), dict(ele_start=5, num_elements=40)):
+        for p in (random_particles(500, seed=9, scale=5e-3), guard_edge_particles(600, seed=10)):
+            _, _, pg, pc = _run(_hybrid_line, p, **kw)
+            bad = compare_exact(pg, pc)
+            assert not bad, bad

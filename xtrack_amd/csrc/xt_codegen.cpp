@@ -49,28 +49,37 @@ constexpr int kSmallCost = 12;
 constexpr size_t kSegMax = 31;           // elements per segment (< 32)
 constexpr int64_t kSegMaxCount = 256;    // distinct segments per line
 constexpr int64_t kSegMaxBody = 2048;    // sum of their lengths
+// line code: every record literal up to this many distinct records, else
+// literal only for records used more than once whose structure occurs with
+// at most kLiteralPerStruct constant sets, at most kLiteralMax of them
+constexpr int64_t kLiteralAll = 64;
+constexpr int64_t kLiteralMax = 1024;
+constexpr int64_t kLiteralPerStruct = 4;
 static_assert(kSegMax <= XT_KEY_SPAN_MASK, "segment length must fit its key field");
 
 struct Gen {
     const std::vector<double> &prog;
     const int64_t base;   // program offset of the record
     const bool loads;     // constants as loads rp[k] instead of literals
+    const bool rel;       // program addresses relative to rp = prog + base
     std::string out;
     std::string err;
     int cost = 0;         // rough size of the record's arithmetic (ops, kick terms)
     bool heavy = false;   // holds an integrator, a map or an RF op
 
-    Gen(const std::vector<double> &p, int64_t b, bool l) : prog(p), base(b), loads(l) {}
+    Gen(const std::vector<double> &p, int64_t b, bool l, bool r = false)
+        : prog(p), base(b), loads(l), rel(l || r) {}
 
     // constant at absolute program index i
     std::string c(int64_t i) const {
         if (loads) return "rp[" + std::to_string(i - base) + "]";
         return lit(prog[i]);
     }
-    // address of program word i: relative to the record in loads mode, so
-    // that records of the same structure produce the same text
+    // address of program word i: relative to the record (rp) in loads mode
+    // and in relative literal mode, so that records of the same structure
+    // (or the same constants) in different places produce the same text
     std::string addr(int64_t i) const {
-        if (loads) return "rp + " + std::to_string(i - base);
+        if (rel) return "rp + " + std::to_string(i - base);
         return "prog + " + std::to_string(i);
     }
     std::string cs(int64_t i, int n) const {
@@ -111,7 +120,7 @@ struct Gen {
         const int64_t mask = wbits(a[2]);
         if (!in_range(desc, KD_ARR)) return false;
         line("if (!xt::kick_sane(p)) {");
-        line("    xt::kick_literal_apply(p, " + w + ", " + addr(desc) + ");");
+        line("    xt::kick_literal_apply(p, " + w + ", (const double *)(" + addr(desc) + "));");
         line("} else {");
         line("    const double x = p.x, y = p.y;");
         for (int k = 0; k < 3; k++) {
@@ -176,8 +185,8 @@ struct Gen {
             case OP_S_ROT: line("xt::op_s_rot(p" + cs(a, 2) + ");"); break;
             case OP_RF_MULT:
                 // the descriptor holds program offsets: absolute addressing
-                line("xt::op_rf_mult(p, m, prog + " + std::to_string(pc + 1) +
-                     ", prog, c.q0, c.kill_energy());");
+                line("xt::op_rf_mult(p, m, xt::cptr(prog) + " + std::to_string(pc + 1) +
+                     ", xt::cptr(prog), c.q0, c.kill_energy());");
                 break;
             default: return false;
             }
@@ -189,160 +198,264 @@ struct Gen {
 
 }  // namespace
 
-int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
-                            int32_t waves_per_eu, uint32_t options, std::string &src,
-                            std::vector<int32_t> &rec_of_elem, std::string &err) {
-    std::map<int64_t, int32_t> index;
-    for (int64_t e : entry) index.emplace(e, 0);
-    if ((int64_t)index.size() > XT_CODEGEN_MAX_RECORDS) {
-        err = "line has " + std::to_string(index.size()) + " distinct element records (more than " +
-              std::to_string(XT_CODEGEN_MAX_RECORDS) + ")";
-        return XTM_ERR_UNSUPPORTED;
-    }
-    int32_t k = 0;
-    for (auto &kv : index) kv.second = k++;
-    rec_of_elem.resize(entry.size());
-    for (size_t e = 0; e < entry.size(); e++) rec_of_elem[e] = index[entry[e]];
+namespace {
 
+/* Functions of the generated code for a set of lines (one line, or the
+   lines of a batch).  A record whose literal code serves at least two
+   elements of the set (every record, for a single line of at most
+   kLiteralAll distinct records) gets a literal function; a record that
+   serves one element only -- a magnet with its own field errors -- gets the
+   function of its structure, shared with every record of that structure,
+   which reads its constants from the program (scalar loads at
+   rp = prog + entry[e]).  Literal functions for thousands of single-use
+   records would be megabytes of instructions, streamed through the
+   instruction cache once per turn. */
+struct CodeSet {
+    std::vector<std::string> body;             // function bodies
+    std::vector<char> small;                   // cheap literal records
+    bool keyed_pc = false;                     // keys carry the program offset
+    std::vector<std::vector<int32_t>> segs;    // segment function bodies
+};
+
+int build_code_set(const std::vector<const std::vector<int64_t> *> &entries,
+                   const std::vector<const std::vector<double> *> &progs, CodeSet &cs,
+                   std::vector<std::vector<int32_t>> &recs, std::string &err) {
+    const size_t nl = entries.size();
+    // distinct records of each line and their element counts
+    std::vector<std::map<int64_t, int64_t>> uses(nl);
+    int64_t n_distinct = 0;
+    for (size_t l = 0; l < nl; l++) {
+        for (int64_t e : *entries[l]) uses[l][e]++;
+        n_distinct += (int64_t)uses[l].size();
+    }
+    const bool all_literal = nl == 1 && n_distinct <= kLiteralAll;
+    cs.keyed_pc = !all_literal;
+    // literal text of every record (program addresses relative to the
+    // record unless every record of a single line is literal)
+    std::map<std::string, int64_t> lit_uses;
+    std::vector<std::map<int64_t, std::string>> lit_text(nl);
+    std::map<std::string, std::pair<bool, int>> lit_info;   // heavy, cost
+    std::map<std::string, std::string> struct_of_lit;
+    std::map<std::string, int64_t> lits_of_struct;         // distinct literal texts
+    for (size_t l = 0; l < nl; l++)
+        for (auto &pu : uses[l]) {
+            Gen g(*progs[l], pu.first, false, !all_literal);
+            Gen gs(*progs[l], pu.first, true);
+            if (!g.record(pu.first) || !gs.record(pu.first)) {
+                err = "cannot generate code for line " + std::to_string(l) +
+                      " at program offset " + std::to_string(pu.first);
+                return XTM_ERR_ARG;
+            }
+            lit_uses[g.out] += pu.second;
+            lit_info[g.out] = {g.heavy, g.cost};
+            lit_text[l][pu.first] = g.out;
+            if (struct_of_lit.emplace(g.out, gs.out).second) lits_of_struct[gs.out]++;
+        }
+    // A structure that the set holds with many different constants (the
+    // bends of ring variants with different cell counts) gets one structure
+    // function rather than a literal copy per constant set.
+    std::vector<std::pair<int64_t, const std::string *>> ranked;
+    for (auto &kv : lit_uses)
+        if (all_literal ||
+            (kv.second >= 2 && lits_of_struct[struct_of_lit[kv.first]] <= kLiteralPerStruct))
+            ranked.emplace_back(kv.second, &kv.first);
+    std::stable_sort(ranked.begin(), ranked.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    if ((int64_t)ranked.size() > kLiteralMax) ranked.resize(kLiteralMax);
+    std::map<std::string, int32_t> fn_of_lit;
+    for (auto &r : ranked) {
+        fn_of_lit.emplace(*r.second, (int32_t)cs.body.size());
+        cs.body.push_back(*r.second);
+        const auto &info = lit_info[*r.second];
+        cs.small.push_back(!info.first && info.second <= kSmallCost);
+    }
+    // the other records: structure functions
+    std::map<std::string, int32_t> fn_of_struct;
+    recs.assign(nl, {});
+    for (size_t l = 0; l < nl; l++) {
+        std::map<int64_t, int32_t> fn_of_pc;
+        for (auto &pu : uses[l]) {
+            auto it = fn_of_lit.find(lit_text[l][pu.first]);
+            if (it != fn_of_lit.end()) {
+                fn_of_pc.emplace(pu.first, it->second);
+                continue;
+            }
+            Gen g(*progs[l], pu.first, true);
+            if (!g.record(pu.first)) {
+                err = "cannot generate code for line " + std::to_string(l) +
+                      " at program offset " + std::to_string(pu.first);
+                return XTM_ERR_ARG;
+            }
+            auto ins = fn_of_struct.emplace(g.out, (int32_t)cs.body.size());
+            if (ins.second) {
+                cs.body.push_back(g.out);
+                cs.small.push_back(0);
+            }
+            fn_of_pc.emplace(pu.first, ins.first->second);
+        }
+        if ((int64_t)cs.body.size() > XT_CODEGEN_MAX_RECORDS) {
+            err = "the code needs more than " + std::to_string(XT_CODEGEN_MAX_RECORDS) +
+                  " element functions";
+            return XTM_ERR_UNSUPPORTED;
+        }
+        const std::vector<int64_t> &entry = *entries[l];
+        recs[l].resize(entry.size());
+        for (size_t e = 0; e < entry.size(); e++) recs[l][e] = fn_of_pc[entry[e]];
+    }
+
+    // Segments: runs of consecutive cheap literal records (drifts,
+    // apertures, thin kicks, markers) cut into chunks of at most kSegMax
+    // elements; each chunk sequence that occurs at least twice in the set
+    // becomes one function that tracks the chunk's elements in order and
+    // returns the position of the element where the particle was lost.  The
+    // key of a chunk's first element carries the segment number and the
+    // chunk length; the loop runs the chunk with one dispatch instead of one
+    // per element.  Same element calls, same order: the arithmetic is that
+    // of the single-record path.  (Cheap records are literal: their code
+    // reads no constants through rp, which the segment call passes for the
+    // chunk's first element only.)
+    std::vector<std::tuple<size_t, size_t, size_t>> chunks;   // (line, first element, length)
+    for (size_t l = 0; l < nl; l++) {
+        const std::vector<int32_t> &rv = recs[l];
+        size_t e = 0;
+        while (e < rv.size()) {
+            if (!cs.small[rv[e]]) { e++; continue; }
+            size_t f = e;
+            while (f < rv.size() && cs.small[rv[f]]) f++;
+            for (size_t a = e; a < f; a += kSegMax) {
+                const size_t n = std::min<size_t>(kSegMax, f - a);
+                if (n > 1) chunks.emplace_back(l, a, n);
+            }
+            e = f;
+        }
+    }
+    auto chunk_seq = [&](const std::tuple<size_t, size_t, size_t> &ch) {
+        const std::vector<int32_t> &rv = recs[std::get<0>(ch)];
+        return std::vector<int32_t>(rv.begin() + std::get<1>(ch),
+                                    rv.begin() + std::get<1>(ch) + std::get<2>(ch));
+    };
+    std::map<std::vector<int32_t>, int64_t> count;
+    for (auto &ch : chunks) count[chunk_seq(ch)]++;
+    std::vector<std::pair<int64_t, const std::vector<int32_t> *>> seg_rank;
+    for (auto &kv : count)
+        if (kv.second >= 2)
+            seg_rank.emplace_back(kv.second * (int64_t)(kv.first.size() - 1), &kv.first);
+    std::stable_sort(seg_rank.begin(), seg_rank.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::map<std::vector<int32_t>, int32_t> seg_id;
+    int64_t seg_body = 0;
+    for (auto &r : seg_rank) {
+        if ((int64_t)cs.segs.size() >= kSegMaxCount ||
+            seg_body + (int64_t)r.second->size() > kSegMaxBody)
+            continue;
+        seg_id.emplace(*r.second, (int32_t)cs.segs.size());
+        cs.segs.push_back(*r.second);
+        seg_body += (int64_t)r.second->size();
+    }
+    for (auto &ch : chunks) {
+        auto it = seg_id.find(chunk_seq(ch));
+        if (it == seg_id.end()) continue;
+        recs[std::get<0>(ch)][std::get<1>(ch)] |= ((it->second + 1) << XT_KEY_SEG_SHIFT) |
+                                                  ((int32_t)std::get<2>(ch) << XT_KEY_SPAN_SHIFT);
+    }
+    return XTM_OK;
+}
+
+/* The functions, segments and element runner of a code set, in namespace
+   xt_gen. */
+std::string emit_code_set(const CodeSet &cs, uint32_t options, const char *what) {
     std::string s;
-    s += "// generated line code: element records as straight-line calls\n";
+    s += std::string("// generated ") + what + ": element records as straight-line calls\n";
     if (options & XTM_SPEC_FAST_ARITH) s += "#define XTM_ARITH_FAST 1\n";
     s += "#if defined(__HIPCC__) && !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
     s += "#include \"xt_ops.h\"\n\n";
     s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";
     s += "#else\n#define XT_GEN_FN static inline\n#endif\n\n";
     s += "namespace xt_gen {\n";
-#ifdef XTM_CODEGEN_CONST_LOADS
-    const bool loads = true;
-#else
-    const bool loads = false;
-#endif
-    std::vector<char> small(k, 0);
-    for (auto &kv : index) {
-        Gen g(prog, kv.first, loads);
-        if (!g.record(kv.first)) {
-            err = "cannot generate code for the program at offset " + std::to_string(kv.first);
-            return XTM_ERR_ARG;
-        }
-        small[kv.second] = !g.heavy && g.cost <= kSmallCost;
-        s += "XT_GEN_FN void rec_" + std::to_string(kv.second) +
+    for (size_t r = 0; r < cs.body.size(); r++) {
+        s += "XT_GEN_FN void rec_" + std::to_string(r) +
              "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
-             "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n"
+             "    xt::CProg rp, const xt::TrackCtx &c) {\n"
              "    (void)m; (void)prog; (void)rp; (void)c;\n";
-        s += g.out;
+        s += cs.body[r];
         s += "}\n\n";
     }
-
-    // Segments: runs of consecutive cheap records (drifts, apertures, thin
-    // kicks, markers) cut into chunks of at most kSegMax elements; each
-    // chunk sequence that occurs at least twice in the line becomes one
-    // function that tracks the chunk's elements in order and returns the
-    // position of the element where the particle was lost.  The key of a
-    // chunk's first element carries the segment number and the chunk
-    // length; the loop runs the chunk with one dispatch instead of one per
-    // element.  Same element calls, same order: the arithmetic is that of
-    // the single-record path.
-    std::vector<std::vector<int32_t>> segs;
-    if (!loads) {
-        std::vector<std::pair<size_t, size_t>> chunks;   // (first element, length)
-        size_t e = 0;
-        while (e < rec_of_elem.size()) {
-            if (!small[rec_of_elem[e]]) { e++; continue; }
-            size_t f = e;
-            while (f < rec_of_elem.size() && small[rec_of_elem[f]]) f++;
-            for (size_t a = e; a < f; a += kSegMax) {
-                const size_t n = std::min<size_t>(kSegMax, f - a);
-                if (n > 1) chunks.emplace_back(a, n);
-            }
-            e = f;
+    // a segment's records that address the program (relative literal code
+    // of thin kicks: the literal kick's descriptor) get their own element's
+    // rp from ent, the entry offsets from the chunk's first element on
+    for (size_t q = 0; q < cs.segs.size(); q++) {
+        s += "XT_GEN_FN int32_t seg_" + std::to_string(q) +
+             "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
+             "    xt::CProg rp, const xt::TrackCtx &c, xt::CP<int64_t> ent) {\n"
+             "    (void)ent;\n";
+        const auto &v = cs.segs[q];
+        for (size_t j = 0; j < v.size(); j++) {
+            const std::string &b = cs.body[v[j]];
+            const bool own_rp = cs.keyed_pc && (b.find("rp[") != std::string::npos ||
+                                                b.find("rp + ") != std::string::npos);
+            s += "    rec_" + std::to_string(v[j]) + "(p, m, prog, " +
+                 (own_rp ? "xt::cptr(prog) + ent[" + std::to_string(j) + "]" : std::string("rp")) +
+                 ", c);";
+            s += j + 1 < v.size() ? " if (p.state <= 0) return " + std::to_string(j) + ";\n"
+                                  : " return " + std::to_string(j) + ";\n";
         }
-        std::map<std::vector<int32_t>, int64_t> count;
-        for (auto &ch : chunks)
-            count[std::vector<int32_t>(rec_of_elem.begin() + ch.first,
-                                       rec_of_elem.begin() + ch.first + ch.second)]++;
-        std::vector<std::pair<int64_t, const std::vector<int32_t> *>> ranked;
-        for (auto &kv : count)
-            if (kv.second >= 2)
-                ranked.emplace_back(kv.second * (int64_t)(kv.first.size() - 1), &kv.first);
-        std::stable_sort(ranked.begin(), ranked.end(),
-                         [](const auto &a, const auto &b) { return a.first > b.first; });
-        std::map<std::vector<int32_t>, int32_t> seg_id;
-        int64_t body = 0;
-        for (auto &r : ranked) {
-            if ((int64_t)segs.size() >= kSegMaxCount ||
-                body + (int64_t)r.second->size() > kSegMaxBody)
-                continue;
-            seg_id.emplace(*r.second, (int32_t)segs.size());
-            segs.push_back(*r.second);
-            body += (int64_t)r.second->size();
-        }
-        for (auto &ch : chunks) {
-            auto it = seg_id.find(std::vector<int32_t>(rec_of_elem.begin() + ch.first,
-                                                       rec_of_elem.begin() + ch.first + ch.second));
-            if (it == seg_id.end()) continue;
-            rec_of_elem[ch.first] |= ((it->second + 1) << XT_KEY_SEG_SHIFT) |
-                                     ((int32_t)ch.second << XT_KEY_SPAN_SHIFT);
-        }
-        for (size_t q = 0; q < segs.size(); q++) {
-            s += "XT_GEN_FN int32_t seg_" + std::to_string(q) +
-                 "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
-                 "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n";
-            const auto &v = segs[q];
-            for (size_t j = 0; j < v.size(); j++) {
-                s += "    rec_" + std::to_string(v[j]) + "(p, m, prog, rp, c);";
-                s += j + 1 < v.size() ? " if (p.state <= 0) return " + std::to_string(j) + ";\n"
-                                      : " return " + std::to_string(j) + ";\n";
-            }
-            s += "}\n\n";
-        }
+        s += "}\n\n";
     }
+    // element key: the function index with the segment fields (int32), and
+    // with relative addressing also the element's program offset
+    const std::string kr = cs.keyed_pc ? "xt::uniform(k.r)" : "xt::uniform(k)";
     s += "struct Runner {\n"
          "    const int32_t *__restrict__ rec;\n"
          "    const int64_t *__restrict__ entry;\n"
          "    const double *__restrict__ prog;\n";
-    if (loads)
+    if (cs.keyed_pc)
         s += "    struct Key { int32_t r; int64_t pc; };\n"
-             "    static constexpr bool kSegments = false;\n"
-             "    XT_HD static int32_t span(Key) { return 0; }\n"
-             "    XT_HD int32_t segment(xt::PState<double> &, const xt::PRef &, Key,\n"
-             "                          const xt::TrackCtx &) const { return 0; }\n"
-             "    XT_HD Key key(int64_t e) const { return Key{rec[e], entry[e]}; }\n"
-             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
-             "                          const xt::TrackCtx &c) const {\n"
-             "        const double *__restrict__ rp = prog + k.pc;\n"
-             "        switch (k.r) {\n";
+             "    XT_HD Key key(int64_t e) const { return Key{xt::cptr(rec)[e], xt::cptr(entry)[e]}; }\n";
     else
         s += "    using Key = int32_t;\n"
-             "    static constexpr bool kSegments = true;\n"
-             "    XT_HD Key key(int64_t e) const { return rec[e]; }\n"
-             "    XT_HD static int32_t span(Key k) { return (k >> " +
-             std::to_string(XT_KEY_SPAN_SHIFT) + ") & " + std::to_string(XT_KEY_SPAN_MASK) +
-             "; }\n"
-             "    XT_HD int32_t segment(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
-             "                          const xt::TrackCtx &c) const {\n"
-             "        const double *__restrict__ rp = prog;\n"
-             "        switch (k >> " + std::to_string(XT_KEY_SEG_SHIFT) + ") {\n" + [&] {
-                 std::string t;
-                 for (size_t q = 0; q < segs.size(); q++)
-                     t += "        case " + std::to_string(q + 1) + ": return seg_" +
-                          std::to_string(q) + "(p, m, prog, rp, c);\n";
-                 return t;
-             }() +
-             "        default: p.state = -999; return 0;\n"
-             "        }\n"
-             "    }\n"
-             "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
-             "                          const xt::TrackCtx &c) const {\n"
-             "        const double *__restrict__ rp = prog;\n"
-             "        switch (k & " + std::to_string(XT_KEY_REC_MASK) + ") {\n";
-    for (int32_t r = 0; r < k; r++)
+             "    XT_HD Key key(int64_t e) const { return xt::cptr(rec)[e]; }\n";
+    const std::string rp = cs.keyed_pc
+                               ? "        const xt::CProg rp = xt::cptr(prog) + xt::uniform(k.pc);\n"
+                               : "        const xt::CProg rp = xt::cptr(prog);\n";
+    s += "    static constexpr bool kSegments = true;\n"
+         "    XT_HD static int32_t span(Key k) { return (" + kr + " >> " +
+         std::to_string(XT_KEY_SPAN_SHIFT) + ") & " + std::to_string(XT_KEY_SPAN_MASK) +
+         "; }\n"
+         "    XT_HD int32_t segment(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
+         "                          const xt::TrackCtx &c, int64_t e) const {\n" + rp +
+         "        const xt::CP<int64_t> ent = xt::cptr(entry) + xt::uniform(e);\n"
+         "        switch (" + kr + " >> " + std::to_string(XT_KEY_SEG_SHIFT) + ") {\n";
+    for (size_t q = 0; q < cs.segs.size(); q++)
+        s += "        case " + std::to_string(q + 1) + ": return seg_" + std::to_string(q) +
+             "(p, m, prog, rp, c, ent);\n";
+    s += "        default: p.state = -999; return 0;\n"
+         "        }\n"
+         "    }\n"
+         "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
+         "                          const xt::TrackCtx &c) const {\n" + rp +
+         "        switch (" + kr + " & " + std::to_string(XT_KEY_REC_MASK) + ") {\n";
+    for (size_t r = 0; r < cs.body.size(); r++)
         s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
              "(p, m, prog, rp, c); break;\n";
     s += "        default: p.state = -999; break;\n"
          "        }\n"
          "    }\n"
-         "};\n"
-         "}  // namespace xt_gen\n\n";
+         "};\n";
+    return s;
+}
+
+}  // namespace
+
+int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector<double> &prog,
+                            int32_t waves_per_eu, uint32_t options, std::string &src,
+                            std::vector<int32_t> &rec_of_elem, std::string &err) {
+    CodeSet cs;
+    std::vector<std::vector<int32_t>> recs;
+    const int st = build_code_set({&entry}, {&prog}, cs, recs, err);
+    if (st != XTM_OK) return st;
+    rec_of_elem = std::move(recs[0]);
+    std::string s = emit_code_set(cs, options, "line code");
+    s += "}  // namespace xt_gen\n\n";
     s += "#if defined(__HIPCC__)\n";
     char occ[128];
     std::snprintf(occ, sizeof occ, "__attribute__((amdgpu_waves_per_eu(%d)))",
@@ -383,81 +496,20 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
     return XTM_OK;
 }
 
+
 int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &entries,
                              const std::vector<const std::vector<double> *> &progs,
                              int32_t waves_per_eu, std::string &src,
                              std::vector<std::vector<int32_t>> &recs, std::string &err) {
-    // one function per distinct structure: the record code of loads mode
-    // (constants read at rp = prog + entry[e]), deduplicated by its text
-    std::map<std::string, int32_t> by_text;
-    std::vector<std::string> bodies;
-    recs.assign(entries.size(), {});
-    for (size_t l = 0; l < entries.size(); l++) {
-        const std::vector<int64_t> &entry = *entries[l];
-        const std::vector<double> &prog = *progs[l];
-        std::map<int64_t, int32_t> of_pc;
-        recs[l].resize(entry.size());
-        for (size_t e = 0; e < entry.size(); e++) {
-            auto it = of_pc.find(entry[e]);
-            if (it == of_pc.end()) {
-                Gen g(prog, entry[e], true);
-                if (!g.record(entry[e])) {
-                    err = "cannot generate code for line " + std::to_string(l) +
-                          " at program offset " + std::to_string(entry[e]);
-                    return XTM_ERR_ARG;
-                }
-                auto ins = by_text.emplace(g.out, (int32_t)bodies.size());
-                if (ins.second) {
-                    bodies.push_back(g.out);
-                    if ((int64_t)bodies.size() > XT_CODEGEN_MAX_RECORDS) {
-                        err = "the lines have more than " +
-                              std::to_string(XT_CODEGEN_MAX_RECORDS) + " distinct element structures";
-                        return XTM_ERR_UNSUPPORTED;
-                    }
-                }
-                it = of_pc.emplace(entry[e], ins.first->second).first;
-            }
-            recs[l][e] = it->second;
-        }
+    CodeSet cs;
+    const int st = build_code_set(entries, progs, cs, recs, err);
+    if (st != XTM_OK) return st;
+    if (!cs.keyed_pc) {
+        err = "internal: batch code without program offsets";
+        return XTM_ERR_ARG;
     }
-    std::string s;
-    s += "// generated batch code: element structures as straight-line calls, constants\n"
-         "// read from each line's program\n";
-    s += "#if defined(__HIPCC__) && !defined(__HIPCC_RTC__)\n#include <hip/hip_runtime.h>\n#endif\n";
-    s += "#include \"xt_ops.h\"\n\n";
-    s += "#if defined(__HIPCC__)\n#define XT_GEN_FN static __host__ __device__ __forceinline__\n";
-    s += "#else\n#define XT_GEN_FN static inline\n#endif\n\n";
-    s += "namespace xt_gen {\n";
-    for (size_t r = 0; r < bodies.size(); r++) {
-        s += "XT_GEN_FN void rec_" + std::to_string(r) +
-             "(xt::PState<double> &p, const xt::PRef &m, const double *__restrict__ prog,\n"
-             "    const double *__restrict__ rp, const xt::TrackCtx &c) {\n"
-             "    (void)m; (void)prog; (void)rp; (void)c;\n";
-        s += bodies[r];
-        s += "}\n\n";
-    }
-    s += "struct Runner {\n"
-         "    const int32_t *__restrict__ rec;\n"
-         "    const int64_t *__restrict__ entry;\n"
-         "    const double *__restrict__ prog;\n"
-         "    struct Key { int32_t r; int64_t pc; };\n"
-         "    static constexpr bool kSegments = false;\n"
-         "    XT_HD static int32_t span(Key) { return 0; }\n"
-         "    XT_HD int32_t segment(xt::PState<double> &, const xt::PRef &, Key,\n"
-         "                          const xt::TrackCtx &) const { return 0; }\n"
-         "    XT_HD Key key(int64_t e) const { return Key{rec[e], entry[e]}; }\n"
-         "    XT_HD void operator()(xt::PState<double> &p, const xt::PRef &m, Key k,\n"
-         "                          const xt::TrackCtx &c) const {\n"
-         "        const double *__restrict__ rp = prog + k.pc;\n"
-         "        switch (k.r) {\n";
-    for (size_t r = 0; r < bodies.size(); r++)
-        s += "        case " + std::to_string(r) + ": rec_" + std::to_string(r) +
-             "(p, m, prog, rp, c); break;\n";
-    s += "        default: p.state = -999; break;\n"
-         "        }\n"
-         "    }\n"
-         "};\n"
-         "/* one block of a batch: particles [first, first + count) of line `line` */\n"
+    std::string s = emit_code_set(cs, 0, "batch code");
+    s += "/* one block of a batch: particles [first, first + count) of line `line` */\n"
          "struct BatchBlock { int64_t first; int32_t line; int32_t count; };\n"
          "}  // namespace xt_gen\n\n";
     char occ[128];
@@ -471,15 +523,16 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
          "    const double *const *__restrict__ progs, const int64_t *__restrict__ nele,\n"
          "    const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,\n"
          "    int32_t num_turns, uint64_t flags) {\n"
-         "    const xt_gen::BatchBlock b = blocks[blockIdx.x];\n"
+         "    const xt_gen::BatchBlock b = xt::cptr(blocks)[blockIdx.x];\n"
          "    if ((int32_t)threadIdx.x >= b.count) return;\n"
          "    const xtm_particles &P = *Pd;\n"
          "    const int64_t i = b.first + threadIdx.x;\n"
-         "    const int64_t ne = nele[b.line];\n"
-         "    const xt::TrackCtx c{flags, lengths[b.line], P.q0, P.t_sim, ne};\n"
+         "    const int64_t ne = xt::cptr(nele)[b.line];\n"
+         "    const xt::TrackCtx c{flags, xt::cptr(lengths)[b.line], P.q0, P.t_sim, ne};\n"
          "    // chi == 1 particles; the interpreter batch kernel tracks the others\n"
-         "    xt::track_particle_chi1(P, i, xt_gen::Runner{recs[b.line], entries[b.line],\n"
-         "                            progs[b.line]}, c, num_turns, 0, (int32_t)ne, 1, 1, 0,\n"
+         "    xt::track_particle_chi1(P, i, xt_gen::Runner{xt::cptr(recs)[b.line],\n"
+         "                            xt::cptr(entries)[b.line], xt::cptr(progs)[b.line]}, c,\n"
+         "                            num_turns, 0, (int32_t)ne, 1, 1, 0,\n"
          "                            xt::MonRef{nullptr, Pd});\n"
          "}\n"
          "#else\n"

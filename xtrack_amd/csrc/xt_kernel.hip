@@ -92,15 +92,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,
     int32_t num_turns, uint64_t flags, int32_t skip_chi1) {
     static_assert(BS <= BLOCK, "the per-thread LDS slots are laid out for 256 threads");
-    const BatchBlock b = blocks[blockIdx.x];
+    const BatchBlock b = xt::cptr(blocks)[blockIdx.x];
     if ((int32_t)threadIdx.x >= b.count) return;
     const xtm_particles &P = *Pd;
     const int64_t i = b.first + threadIdx.x;
     // after the compiled batch kernel: only the particles it left (chi != 1)
     if (skip_chi1 && P.chi[i] == 1.0) return;
-    const int64_t ne = nele[b.line];
-    const xt::TrackCtx c{flags, lengths[b.line], P.q0, P.t_sim, ne};
-    const xt::ProgramRunner run{entries[b.line], progs[b.line]};
+    const int64_t ne = xt::cptr(nele)[b.line];
+    const xt::TrackCtx c{flags, xt::cptr(lengths)[b.line], P.q0, P.t_sim, ne};
+    const xt::ProgramRunner run{xt::cptr(entries)[b.line], xt::cptr(progs)[b.line]};
     xt::track_particle_with<false>(P, i, run, run, c, num_turns, 0, (int32_t)ne, 1, 1, 0,
                                    xt::MonRef{nullptr, Pd});
 }

@@ -112,6 +112,50 @@ struct PRef {
 
 XT_HD int64_t wbits(double d) { int64_t u; __builtin_memcpy(&u, &d, 8); return u; }
 
+/* Element programs, entry offsets and record keys are read-only for the
+   whole launch.  Reading them through the constant address space (4) lets
+   the compiler load wave-uniform words with scalar loads into SGPRs
+   through the scalar cache; through a generic pointer it cannot rule out
+   that the kernel's own stores change them and loads every constant into
+   vector registers, once per lane.  CP<T> is the pointer type for such
+   reads; gptr converts back for the out-of-line functions. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define XT_CONST_AS __attribute__((address_space(4)))
+#else
+#define XT_CONST_AS
+#endif
+template <class T>
+using CP = const XT_CONST_AS T *;
+using CProg = CP<double>;
+template <class T>
+XT_HD CP<T> cptr(const T *p) { return (CP<T>)p; }
+template <class T>
+XT_HD const T *gptr(CP<T> p) { return (const T *)p; }
+
+/* An element key (record index, program offset) is the same on every
+   active lane of a wave: the lanes walk the elements together and a lost
+   lane leaves the loop.  The compiler cannot always prove it -- a key
+   carried out of a loop that lanes leave at different elements counts as
+   divergent -- and would then address the element's constants with vector
+   loads and dispatch on it lane by lane.  uniform() states it: the first
+   active lane's value, in scalar registers. */
+XT_HD int32_t uniform(const int32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+XT_HD int64_t uniform(const int64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+#else
+    return v;
+#endif
+}
+
 /* elementary functions for double; the counting scalar overloads these */
 XT_HD double xsqrt(double v) { return XT_SQRT(v); }
 XT_HD void xsincos(double v, double *s, double *c) { xt_sincos(v, s, c); }
@@ -667,11 +711,10 @@ XT_HD void kick_curvature(PState<R> &p, const R x, const R y, const bool has_hl,
 }
 
 /* Horner pass on a coefficient block [top, a[0..top], b[0..top]] */
-template <class R>
-XT_HD void kick_pass_fast(PState<R> &p, const double *__restrict__ fp, double kick_weight,
-                          R x, R y) {
+template <class R, class PP>
+XT_HD void kick_pass_fast(PState<R> &p, const PP fp, double kick_weight, R x, R y) {
     const int64_t top = wbits(fp[0]);
-    const double *a = fp + 1, *b = fp + 2 + top;
+    const PP a = fp + 1, b = fp + 2 + top;
     R dpx_mul = R(a[top]);
     R dpy_mul = R(b[top]);
     for (int64_t k = top - 1; k >= 0; k--)
@@ -681,13 +724,13 @@ XT_HD void kick_pass_fast(PState<R> &p, const double *__restrict__ fp, double ki
 
 /* OP_KICK args: w desc mask hl g0 h1; the descriptor holds the literal
    parameters and the offsets of the three coefficient blocks */
-template <class R>
-XT_HD void magnet_kick(PState<R> &p, const double *__restrict__ a, const double *__restrict__ prog) {
+template <class R, class PP>
+XT_HD void magnet_kick(PState<R> &p, const PP a, const PP prog) {
     const double w = a[0];
-    const double *__restrict__ d = prog + wbits(a[1]);
+    const PP d = prog + wbits(a[1]);
     const int64_t mask = wbits(a[2]);
     if (!kick_sane(p)) {
-        kick_literal_apply(p, w, d);
+        kick_literal_apply(p, w, (const double *)d);
         return;
     }
     const R x = p.x, y = p.y;
@@ -880,14 +923,14 @@ XT_HD void op_yoshida4(PState<R> &p, const double L, const double rho, const dou
 }
 
 /* track_magnet_drift.h:510-549 (a = the 26 op arguments) */
-template <class R>
-XT_HD void op_yoshida6(PState<R> &p, const double *__restrict__ a) {
+template <class R, class PP>
+XT_HD void op_yoshida6(PState<R> &p, const PP a) {
     const double L = a[0], rho = a[1];
-    const double *kc = a + 2;
-    const double *dd = a + 6;
+    const PP kc = a + 2;
+    const PP dd = a + 6;
     for (int j = 0; j < 8; j++) {
         const int di = j < 4 ? j : 7 - j;
-        const double *s = dd + 5 * di;
+        const PP s = dd + 5 * di;
         polar_drift(p, s[0], rho, s[1], s[2], s[3], s[4]);
         if (j < 7) {
             const int ki = j < 4 ? j : 6 - j;
@@ -915,27 +958,26 @@ XT_HD void op_edge_linear(PState<R> &p, const double r21, const double r43) {
 }
 
 /* a = the six RF arguments */
-template <class R>
-XT_HD void op_rf_kick(PState<R> &p, const PRef &m, const double *__restrict__ a,
-                      const double line_length, const double q0, const double t_sim,
-                      const bool kill_cavity) {
+template <class R, class PP>
+XT_HD void op_rf_kick(PState<R> &p, const PRef &m, const PP a, const double line_length,
+                      const double q0, const double t_sim, const bool kill_cavity) {
     if (kill_cavity) return;
     update_ptau(p,
                 rf_kick_ptau(p.zeta, R(m.P->ptau[m.i]), R(m.P->beta0[m.i]),
-                             R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), p.at_turn, a,
-                             line_length, q0, t_sim),
+                             R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), p.at_turn,
+                             (const double *)a, line_length, q0, t_sim),
                 m);
 }
 
 /* rfmultipole.h:11-47: one kick (model -1); the energy update with
    pz_only = 1 (custom_api.h:196-216) */
-template <class R>
-XT_HD void op_rf_mult(PState<R> &p, const PRef &m, const double *__restrict__ a,
-                      const double *__restrict__ prog, const double q0, const bool kill_cavity) {
+template <class R, class PP>
+XT_HD void op_rf_mult(PState<R> &p, const PRef &m, const PP a, const PP prog, const double q0,
+                      const bool kill_cavity) {
     const RFMOut<R> o = rf_multipole_kick(
         p.x, p.y, p.px, p.py, p.zeta, R(m.P->ptau[m.i]), R(m.P->beta0[m.i]),
         R(m.P->charge_ratio[m.i]), p.chi, R(m.P->p0c[m.i]), a[0], a[1], a[2], a[3],
-        prog + wbits(a[4]), q0, kill_cavity);
+        (const double *)(prog + wbits(a[4])), q0, kill_cavity);
     p.px = o.px;
     p.py = o.py;
     if (!kill_cavity) update_ptau(p, o.ptau, m);
@@ -1075,11 +1117,11 @@ struct TrackCtx {
 
 /* ---- element program interpreter ------------------------------------------ */
 template <class R>
-XT_HD void run_element(PState<R> &p, const PRef &m, const double *__restrict__ prog, int64_t pc,
+XT_HD void run_element(PState<R> &p, const PRef &m, const CProg prog, int64_t pc,
                        const TrackCtx &c) {
     for (;;) {
         const int op = (int)wbits(prog[pc]);
-        const double *__restrict__ a = prog + pc + 1;
+        const CProg a = prog + pc + 1;
         switch (op) {
         case OP_END:
             return;
@@ -1337,7 +1379,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
                 const typename Runner::Key next = run.key(e + n < ele_stop ? e + n : e);
                 if (flag_monitor == 2) XT_MONITOR_RECORD(base + e);
                 if constexpr (Runner::kSegments) {
-                    if (n > 1) j = run.segment(p, m, key, c);
+                    if (n > 1) j = run.segment(p, m, key, c, e);
                     else run(p, m, key, c);
                 } else {
                     run(p, m, key, c);
@@ -1398,9 +1440,9 @@ struct ProgramRunner {
     const double *__restrict__ prog;
     using Key = int64_t;
     static constexpr bool kSegments = false;
-    XT_HD Key key(int64_t e) const { return entry[e]; }
+    XT_HD Key key(int64_t e) const { return cptr(entry)[e]; }
     XT_HD void operator()(PState<double> &p, const PRef &m, Key pc, const TrackCtx &c) const {
-        run_element(p, m, prog, pc, c);
+        run_element(p, m, cptr(prog), uniform(pc), c);
     }
 };
 

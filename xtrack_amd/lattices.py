@@ -10,6 +10,8 @@ collider_ring: 8 arcs x 23 FODO cells; a cell holds 6 thick Bends
     magnets are drifts split in 6 pieces; a LimitRect (+/-0.022 x
     +/-0.018 m) follows every element; one Cavity (400 MHz, 16 MV, lag 180)
     at the start.  28,338 elements; protons at 7 TeV.
+collider_with_errors: the collider ring with a seeded set of b3..b6
+    multipole errors on every magnet (dynamic-aperture studies).
 """
 import math
 
@@ -104,6 +106,40 @@ def collider_variant(seed, k2f=0.0, k2d=0.0, n_arcs=8, cells_per_arc=(8, 40), re
             ee.k1 = ee.k1 * (1.0 + rel_err * rng.standard_normal())
         elif isinstance(ee, xe.Sextupole):
             ee.k2 = ee.k2 * (1.0 + rel_err * rng.standard_normal())
+    return line
+
+
+def collider_with_errors(seed, k2f=0.0, k2d=0.0, rel=1e-4, r0=0.017, orders=(3, 4, 5, 6),
+                         **kwargs):
+    """The collider ring with a random multipole error set: on every bend,
+    quadrupole and sextupole, each normal multipole b_n of `orders`
+    (European numbering: b3 sextupole ... b6 dodecapole) other than the
+    magnet's main order gets a field error of N(0, rel) relative to the main
+    field at the reference radius r0, i.e.
+
+        knl[n-1] += eps * F_main(r0) * (n-1)! / r0**(n-1),
+
+    with F_main(r0) = angle (bend), k1 L r0 (quadrupole), k2 L r0**2 / 2
+    (sextupole) and eps ~ N(0, rel) from numpy default_rng(seed), drawn
+    magnet by magnet in line order, orders ascending."""
+    rng = np.random.default_rng(seed)
+    line = collider_ring(k2f=k2f, k2d=k2d, **kwargs)
+    for ee in line.elements:
+        if isinstance(ee, xe.Bend):
+            main, f_main = 1, ee.angle
+        elif isinstance(ee, xe.Quadrupole):
+            main, f_main = 2, ee.k1 * ee.length * r0
+        elif isinstance(ee, xe.Sextupole):
+            main, f_main = 3, ee.k2 * ee.length * r0 ** 2 / 2
+        else:
+            continue
+        knl = np.array(ee.knl, dtype=np.float64)
+        for n in orders:
+            if n == main:
+                continue
+            knl[n - 1] += rel * rng.standard_normal() * f_main * math.factorial(n - 1) / \
+                r0 ** (n - 1)
+        ee.knl = knl
     return line
 
 
