@@ -156,8 +156,8 @@ def measure_traffic(args):
                    '--arithmetic', args.arithmetic, '--no-variants', '--no-batch',
                    '--waves-per-eu', str(args.waves_per_eu)]
             log(f'traffic pass {counter}')
-            with open(os.path.join(tmp, counter + '.log'), 'w') as log:
-                subprocess.run(cmd, stdout=log, stderr=subprocess.STDOUT, timeout=420,
+            with open(os.path.join(tmp, counter + '.log'), 'w') as fh:
+                subprocess.run(cmd, stdout=fh, stderr=subprocess.STDOUT, timeout=420,
                                check=True)
             kib_copy, kib_track = _read_counter(outdir, counter)
             scale = copy_bytes / (kib_copy * 1024.0)

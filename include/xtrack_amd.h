@@ -238,6 +238,35 @@ typedef struct xtm_monitor {
     xtm_particles data;
 } xtm_monitor;
 
+/* ---- turn-by-turn feature accumulators (no counterpart in the reference) --
+ * Inputs of the dynamic-aperture surrogate, gathered inside the tracking
+ * kernel at the start of every turn t in [start_at_turn, stop_at_turn) for
+ * each active particle, instead of a full turn-by-turn monitor record:
+ * z = inv_w (x - co_x, px - co_px, y - co_y, py - co_py) scaled by
+ * inv_sigma (normalised coordinates), jx = (zx^2 + zpx^2) / 2, jy likewise,
+ * and the phase advance since the previous recorded turn in each plane
+ * (atan2 of the rotation between consecutive (z, zp), in [0, 2 pi)).
+ * acc holds XTM_FEAT_N x capacity doubles, accumulator k of particle slot i
+ * at acc[k * capacity + i]; zero it before the first recorded turn. */
+#define XTM_FEAT_N 15
+enum {
+    XTM_FEAT_COUNT = 0,      /* turns recorded */
+    XTM_FEAT_JX_SUM = 1, XTM_FEAT_JX_SQ = 2, XTM_FEAT_JX_MAX = 3,
+    XTM_FEAT_JY_SUM = 4, XTM_FEAT_JY_SQ = 5, XTM_FEAT_JY_MAX = 6,
+    XTM_FEAT_PHX_SUM = 7, XTM_FEAT_PHY_SUM = 8,
+    XTM_FEAT_ZX = 9, XTM_FEAT_ZPX = 10, XTM_FEAT_ZY = 11, XTM_FEAT_ZPY = 12,   /* last turn */
+    XTM_FEAT_JX0 = 13, XTM_FEAT_JY0 = 14
+};
+typedef struct xtm_features {
+    double *acc;
+    int64_t capacity;
+    int64_t start_at_turn;
+    int64_t stop_at_turn;
+    double inv_w[16];        /* row-major 4 x 4 */
+    double co[4];
+    double inv_sigma[2];
+} xtm_features;
+
 /* Opaque device-resident element table for one line. */
 typedef struct xtm_line xtm_line;
 
@@ -331,6 +360,15 @@ int32_t xtm_track_line(xtm_line *line, const xtm_particles *particles,
                        const xtm_monitor *monitor, uint64_t track_flags,
                        void *stream);
 
+/* xtm_track_line over num_turns whole turns from element 0 (end-of-turn
+ * actions on, s reset, no monitor) with the feature accumulators of
+ * `features` updated at the start of each turn (no counterpart in the
+ * reference: the surrogate's feature turns without a turn-by-turn record).
+ * features->capacity must equal the particle set's. */
+int32_t xtm_track_line_features(xtm_line *line, const xtm_particles *particles,
+                                int32_t num_turns, const xtm_features *features,
+                                uint64_t track_flags, void *stream);
+
 /* Operation count of one turn of the kernel's element maps for one
  * particle with 6D coordinates coords = (x, px, y, py, zeta, delta) and
  * chi = 1, obtained by running the same element code on the host with an
@@ -392,6 +430,13 @@ int32_t xtm_batch_generate_source(const xtm_batch *batch, int32_t waves_per_eu, 
  * only (no allocation, no copy), so it may be captured in a HIP graph. */
 int32_t xtm_batch_track(xtm_batch *batch, const xtm_particles *particles,
                         int32_t num_turns, uint64_t track_flags, void *stream);
+
+/* xtm_batch_track with the feature accumulators of `features` (capacity
+ * equal to the particle set's) updated at the start of each turn, as in
+ * xtm_track_line_features; one normalisation for all lines. */
+int32_t xtm_batch_track_features(xtm_batch *batch, const xtm_particles *particles,
+                                 int32_t num_turns, const xtm_features *features,
+                                 uint64_t track_flags, void *stream);
 
 /* ---- survival-turn surrogate network --------------------------------------
  * No counterpart in the reference.  C[M x N] = op(A) op(B) on the matrix

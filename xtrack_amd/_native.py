@@ -40,6 +40,32 @@ class XtmMonitor(ctypes.Structure):
                 ('data', XtmParticles)]
 
 
+XTM_FEAT_N = 15
+
+
+class XtmFeatures(ctypes.Structure):
+    _fields_ = [('acc', ctypes.c_void_p), ('capacity', ctypes.c_int64),
+                ('start_at_turn', ctypes.c_int64), ('stop_at_turn', ctypes.c_int64),
+                ('inv_w', ctypes.c_double * 16), ('co', ctypes.c_double * 4),
+                ('inv_sigma', ctypes.c_double * 2)]
+
+
+def make_features_struct(acc, capacity, start_at_turn, stop_at_turn, inv_w, co, inv_sigma):
+    """XtmFeatures over the accumulator array `acc` (XTM_FEAT_N x capacity
+    doubles, device memory for the GPU)."""
+    st = XtmFeatures()
+    st.acc = _ptr(acc)
+    st.capacity = int(capacity)
+    st.start_at_turn, st.stop_at_turn = int(start_at_turn), int(stop_at_turn)
+    w = [float(v) for v in list(inv_w.reshape(-1))]
+    if len(w) != 16:
+        raise ValueError('inv_w must be 4 x 4')
+    st.inv_w[:] = w
+    st.co[:] = [float(v) for v in co]
+    st.inv_sigma[:] = [float(v) for v in inv_sigma]
+    return st
+
+
 def _ptr(arr):
     """Raw data pointer of a numpy array or torch tensor."""
     if hasattr(arr, 'data_ptr'):
@@ -121,6 +147,9 @@ def load():
     lib.xtm_track_line.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32, c.c_int32,
                                    c.c_int32, c.c_int32, c.c_int32, c.c_int32,
                                    c.POINTER(XtmMonitor), c.c_uint64, c.c_void_p]
+    lib.xtm_track_line_features.restype = c.c_int32
+    lib.xtm_track_line_features.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
+                                            c.POINTER(XtmFeatures), c.c_uint64, c.c_void_p]
     lib.xtm_line_count_ops.restype = c.c_int32
     lib.xtm_line_count_ops.argtypes = [c.c_void_p, c.c_void_p, c.c_double, c.c_double,
                                        c.c_double, c.c_void_p]
@@ -157,6 +186,9 @@ def load():
     lib.xtm_batch_track.restype = c.c_int32
     lib.xtm_batch_track.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
                                     c.c_uint64, c.c_void_p]
+    lib.xtm_batch_track_features.restype = c.c_int32
+    lib.xtm_batch_track_features.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
+                                             c.POINTER(XtmFeatures), c.c_uint64, c.c_void_p]
     lib.xtm_batch_specialize.restype = c.c_int32
     lib.xtm_batch_specialize.argtypes = [c.c_void_p, c.c_int32]
     lib.xtm_batch_is_specialized.restype = c.c_int32

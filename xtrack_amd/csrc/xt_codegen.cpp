@@ -467,14 +467,14 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
          "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
          "    const xtm_monitor *__restrict__ Md, const xtm_particles *__restrict__ Pd,\n"
-         "    uint64_t flags) {\n"
+         "    const xtm_features *__restrict__ Fd, uint64_t flags) {\n"
          "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
          "    // chi == 1 particles; the interpreter kernel tracks the others\n"
          "    xt::track_particle_chi1(P, i, xt_gen::Runner{rec, entry, prog}, c, num_turns,\n"
          "                            ele_start, num_ele_track, flag_end_turn_actions,\n"
-         "                            flag_reset_s, flag_monitor, xt::MonRef{Md, Pd});\n"
+         "                            flag_reset_s, flag_monitor, xt::MonRef{Md, Pd, Fd});\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over all particles\n"
@@ -522,7 +522,7 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
          "    const int32_t *const *__restrict__ recs, const int64_t *const *__restrict__ entries,\n"
          "    const double *const *__restrict__ progs, const int64_t *__restrict__ nele,\n"
          "    const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,\n"
-         "    int32_t num_turns, uint64_t flags) {\n"
+         "    const xtm_features *__restrict__ Fd, int32_t num_turns, uint64_t flags) {\n"
          "    const xt_gen::BatchBlock b = xt::cptr(blocks)[blockIdx.x];\n"
          "    if ((int32_t)threadIdx.x >= b.count) return;\n"
          "    const xtm_particles &P = *Pd;\n"
@@ -533,7 +533,7 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
          "    xt::track_particle_chi1(P, i, xt_gen::Runner{xt::cptr(recs)[b.line],\n"
          "                            xt::cptr(entries)[b.line], xt::cptr(progs)[b.line]}, c,\n"
          "                            num_turns, 0, (int32_t)ne, 1, 1, 0,\n"
-         "                            xt::MonRef{nullptr, Pd});\n"
+         "                            xt::MonRef{nullptr, Pd, Fd});\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over the blocks\n"

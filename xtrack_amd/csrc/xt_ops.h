@@ -1306,7 +1306,59 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
 struct MonRef {
     const xtm_monitor *M;
     const xtm_particles *P;
+    const xtm_features *F = nullptr;   // feature accumulators, or none
 };
+
+/* Feature accumulators of particle slot i at the start of a turn (see
+   xtm_features); out of line with scalar arguments, like the monitor
+   record, so the element loop carries none of its registers. */
+XT_HD_COLD void feature_record(const xtm_features &F, const int64_t i, const double x,
+                               const double px, const double y, const double py,
+                               const int64_t at_turn) {
+    if (at_turn < F.start_at_turn || at_turn >= F.stop_at_turn) return;
+    const double d[4] = {x - F.co[0], px - F.co[1], y - F.co[2], py - F.co[3]};
+    double z[4];
+    for (int r = 0; r < 4; r++)
+        z[r] = F.inv_w[4 * r] * d[0] + F.inv_w[4 * r + 1] * d[1] + F.inv_w[4 * r + 2] * d[2] +
+               F.inv_w[4 * r + 3] * d[3];
+    const double zx = z[0] * F.inv_sigma[0], zpx = z[1] * F.inv_sigma[0];
+    const double zy = z[2] * F.inv_sigma[1], zpy = z[3] * F.inv_sigma[1];
+    const double jx = 0.5 * (zx * zx + zpx * zpx), jy = 0.5 * (zy * zy + zpy * zpy);
+    double *a = F.acc;
+    const int64_t c = F.capacity;
+    double &n = a[XTM_FEAT_COUNT * c + i];
+    if (n == 0.0) {
+        a[XTM_FEAT_JX0 * c + i] = jx;
+        a[XTM_FEAT_JY0 * c + i] = jy;
+        a[XTM_FEAT_JX_MAX * c + i] = jx;
+        a[XTM_FEAT_JY_MAX * c + i] = jy;
+    } else {
+        /* rotation from the previous (z, zp) to this one: the phase
+           advance for z = sqrt(2J) cos(phi), zp = -sqrt(2J) sin(phi) */
+        const double qx = a[XTM_FEAT_ZX * c + i], qpx = a[XTM_FEAT_ZPX * c + i];
+        const double qy = a[XTM_FEAT_ZY * c + i], qpy = a[XTM_FEAT_ZPY * c + i];
+        const double twopi = 2.0 * PI;
+        double px_ = xt_atan2(zx * qpx - zpx * qx, qx * zx + qpx * zpx);
+        double py_ = xt_atan2(zy * qpy - zpy * qy, qy * zy + qpy * zpy);
+        if (px_ < 0.0) px_ += twopi;
+        if (py_ < 0.0) py_ += twopi;
+        a[XTM_FEAT_PHX_SUM * c + i] += px_;
+        a[XTM_FEAT_PHY_SUM * c + i] += py_;
+        double &mx = a[XTM_FEAT_JX_MAX * c + i];
+        double &my = a[XTM_FEAT_JY_MAX * c + i];
+        if (jx > mx) mx = jx;
+        if (jy > my) my = jy;
+    }
+    a[XTM_FEAT_JX_SUM * c + i] += jx;
+    a[XTM_FEAT_JX_SQ * c + i] += jx * jx;
+    a[XTM_FEAT_JY_SUM * c + i] += jy;
+    a[XTM_FEAT_JY_SQ * c + i] += jy * jy;
+    a[XTM_FEAT_ZX * c + i] = zx;
+    a[XTM_FEAT_ZPX * c + i] = zpx;
+    a[XTM_FEAT_ZY * c + i] = zy;
+    a[XTM_FEAT_ZPY * c + i] = zpy;
+    n = n + 1.0;
+}
 
 #define XT_MONITOR_RECORD(AT_ELEMENT)                                                     \
     monitor_record(*mon.M, *mon.P, i, p.x, p.px, p.y, p.py, p.zeta, p.delta(), p.rpp(),       \
@@ -1361,6 +1413,7 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
         int64_t base = P.at_element[i] - ele_start;
         for (int32_t iturn = 0; iturn < num_turns; iturn++) {
             if (flag_monitor == 1) XT_MONITOR_RECORD(base + ele_start);
+            if (mon.F) feature_record(*mon.F, i, p.x, p.px, p.y, p.py, p.at_turn);
             /* the next element's key (record index / program offset) is
                loaded one element ahead, so its scalar load overlaps the
                current element's arithmetic */

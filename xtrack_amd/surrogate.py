@@ -192,6 +192,67 @@ def _take(p, idx):
     return out
 
 
+def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, starts=None):
+    """Track `particles` (a device particle set) for num_turns whole turns
+    with the feature accumulators updated inside the tracking kernel at the
+    start of every turn (xtm_track_line_features / xtm_batch_track_features);
+    returns the accumulators, a (XTM_FEAT_N, capacity) float64 tensor on the
+    device, row k = accumulator k of every particle slot.
+
+    line_or_batch: a Line with a device tracker, or a BatchTracker together
+    with `starts` (the variants' particle ranges in `particles`)."""
+    torch = _t()
+    lib = _native.load()
+    n = particles._capacity
+    acc = torch.zeros((_native.XTM_FEAT_N, n), dtype=torch.float64,
+                      device=particles._arrays['x'].device)
+    t0 = int(particles._arrays['at_turn'].min().item()) if n else 0
+    fst = _native.make_features_struct(acc, n, t0, t0 + num_turns, np.asarray(inv_w4),
+                                       co, (1.0 / sx, 1.0 / sy))
+    pst = _native.make_particles_struct(particles)
+    if starts is None:
+        tr = line_or_batch.tracker
+        _native.check(lib.xtm_track_line_features(
+            tr._handle.ptr, ctypes.byref(pst), int(num_turns), ctypes.byref(fst),
+            ctypes.c_uint64(int(tr.track_flags.get_flags_register())),
+            tr.backend.stream()))
+    else:
+        bt = line_or_batch
+        bt._set_layout(np.asarray(starts, dtype=np.int64), bt.config or (256, 4))
+        _native.check(lib.xtm_batch_track_features(
+            bt._batch, ctypes.byref(pst), int(num_turns), ctypes.byref(fst),
+            ctypes.c_uint64(int(bt.track_flags.get_flags_register())), bt._stream()))
+    return acc
+
+
+def features_from_acc(acc, x_norm0, y_norm0, delta):
+    """The 16 network inputs per particle from the in-kernel accumulators:
+    r, theta, delta, the initial actions, mean, spread and maximum of the
+    normalised actions over the recorded turns (relative to the initial
+    ones), the mean phase advance per turn in each plane (tunes), the
+    fraction of the turns recorded alive, cos and sin of theta."""
+    torch = _t()
+    dev = acc.device
+    n = acc[0].clamp_min(1.0)       # turns recorded
+    nturn = float(acc[0].max().clamp_min(1.0))
+    mjx, mjy = acc[1] / n, acc[4] / n
+    sjx = torch.sqrt((acc[2] / n - mjx * mjx).clamp_min(0.0))
+    sjy = torch.sqrt((acc[5] / n - mjy * mjy).clamp_min(0.0))
+    j0x, j0y = acc[13].clamp_min(1e-12), acc[14].clamp_min(1e-12)
+    nph = (acc[0] - 1.0).clamp_min(1.0)
+    qx, qy = acc[7] / nph / (2 * math.pi), acc[8] / nph / (2 * math.pi)
+    r0 = torch.sqrt(torch.as_tensor(x_norm0 ** 2 + y_norm0 ** 2, device=dev))
+    th0 = torch.atan2(torch.as_tensor(y_norm0, device=dev), torch.as_tensor(x_norm0, device=dev))
+    f = torch.stack([r0, th0, torch.as_tensor(delta * 1e3, device=dev),
+                     torch.log1p(j0x), torch.log1p(j0y),
+                     torch.log(mjx.clamp_min(1e-12) / j0x), torch.log(mjy.clamp_min(1e-12) / j0y),
+                     sjx / mjx.clamp_min(1e-12), sjy / mjy.clamp_min(1e-12),
+                     torch.log(acc[3].clamp_min(1e-12) / j0x),
+                     torch.log(acc[6].clamp_min(1e-12) / j0y),
+                     qx, qy, acc[0] / nturn, torch.cos(th0), torch.sin(th0)], dim=1)
+    return torch.nan_to_num(f, nan=0.0, posinf=50.0, neginf=-50.0).to(torch.float32)
+
+
 def features(mon, x_norm0, y_norm0, delta, invW4, sx, sy, n_turns):
     """16 features per particle from its initial normalised position and
     the turn-by-turn record of its first n_turns turns: r, theta, delta,
