@@ -239,9 +239,9 @@ typedef struct xtm_monitor {
 } xtm_monitor;
 
 /* ---- turn-by-turn feature accumulators (no counterpart in the reference) --
- * Inputs of the dynamic-aperture surrogate, gathered inside the tracking
- * kernel at the start of every turn t in [start_at_turn, stop_at_turn) for
- * each active particle, instead of a full turn-by-turn monitor record:
+ * Inputs of the dynamic-aperture surrogate, gathered on the device at the
+ * start of every turn t in [start_at_turn, stop_at_turn) for each active
+ * particle, instead of a full turn-by-turn monitor record:
  * z = inv_w (x - co_x, px - co_px, y - co_y, py - co_py) scaled by
  * inv_sigma (normalised coordinates), jx = (zx^2 + zpx^2) / 2, jy likewise,
  * and the phase advance since the previous recorded turn in each plane
@@ -360,11 +360,17 @@ int32_t xtm_track_line(xtm_line *line, const xtm_particles *particles,
                        const xtm_monitor *monitor, uint64_t track_flags,
                        void *stream);
 
-/* xtm_track_line over num_turns whole turns from element 0 (end-of-turn
- * actions on, s reset, no monitor) with the feature accumulators of
- * `features` updated at the start of each turn (no counterpart in the
- * reference: the surrogate's feature turns without a turn-by-turn record).
- * features->capacity must equal the particle set's. */
+/* Update the feature accumulators of `features` (capacity equal to the
+ * particle set's) with the present coordinates of every active particle
+ * whose at_turn lies in [start_at_turn, stop_at_turn): one record at the
+ * start of its current turn (no counterpart in the reference). */
+int32_t xtm_features_record(const xtm_particles *particles, const xtm_features *features,
+                            void *stream);
+
+/* num_turns whole turns from element 0 (end-of-turn actions on, s reset,
+ * no monitor) as one-turn xtm_track_line launches, each preceded by
+ * xtm_features_record: the surrogate's feature turns without a
+ * turn-by-turn record (no counterpart in the reference). */
 int32_t xtm_track_line_features(xtm_line *line, const xtm_particles *particles,
                                 int32_t num_turns, const xtm_features *features,
                                 uint64_t track_flags, void *stream);
@@ -431,9 +437,9 @@ int32_t xtm_batch_generate_source(const xtm_batch *batch, int32_t waves_per_eu, 
 int32_t xtm_batch_track(xtm_batch *batch, const xtm_particles *particles,
                         int32_t num_turns, uint64_t track_flags, void *stream);
 
-/* xtm_batch_track with the feature accumulators of `features` (capacity
- * equal to the particle set's) updated at the start of each turn, as in
- * xtm_track_line_features; one normalisation for all lines. */
+/* xtm_batch_track as one-turn launches, each preceded by
+ * xtm_features_record (as xtm_track_line_features; one normalisation for
+ * all lines). */
 int32_t xtm_batch_track_features(xtm_batch *batch, const xtm_particles *particles,
                                  int32_t num_turns, const xtm_features *features,
                                  uint64_t track_flags, void *stream);

@@ -1,5 +1,5 @@
-"""Feature accumulators gathered inside the tracking kernel
-(xtm_track_line_features, xtm_batch_track_features) against the same
+"""Feature accumulators gathered on the device between one-turn tracking
+launches (xtm_track_line_features, xtm_batch_track_features) against the same
 quantities computed in numpy from a turn-by-turn monitor record of the same
 particles: the tracking itself is unchanged (identical bits), and the
 accumulated sums, maxima and phase advances agree to rounding."""
@@ -22,8 +22,7 @@ def _normalisation(line):
     co = find_closed_orbit(line, ref, method='4d')
     R, _ = compute_one_turn_matrix_finite_differences(line, ref, co, kill_cavity=True)
     inv_w4 = get_linear_normal_form(R, only_4d_block=True)[1][:4, :4]
-    c = co.to_numpy()
-    return inv_w4, [float(c[f][0]) for f in ('x', 'px', 'y', 'py')]
+    return inv_w4, [float(v) for v in co[:4]]
 
 
 def _expected(mon, inv_w4, co, sx, sy, n_turns):

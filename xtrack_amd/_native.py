@@ -147,6 +147,9 @@ def load():
     lib.xtm_track_line.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32, c.c_int32,
                                    c.c_int32, c.c_int32, c.c_int32, c.c_int32,
                                    c.POINTER(XtmMonitor), c.c_uint64, c.c_void_p]
+    lib.xtm_features_record.restype = c.c_int32
+    lib.xtm_features_record.argtypes = [c.POINTER(XtmParticles), c.POINTER(XtmFeatures),
+                                        c.c_void_p]
     lib.xtm_track_line_features.restype = c.c_int32
     lib.xtm_track_line_features.argtypes = [c.c_void_p, c.POINTER(XtmParticles), c.c_int32,
                                             c.POINTER(XtmFeatures), c.c_uint64, c.c_void_p]

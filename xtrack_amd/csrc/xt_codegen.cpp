@@ -218,8 +218,8 @@ struct CodeSet {
 };
 
 int build_code_set(const std::vector<const std::vector<int64_t> *> &entries,
-                   const std::vector<const std::vector<double> *> &progs, CodeSet &cs,
-                   std::vector<std::vector<int32_t>> &recs, std::string &err) {
+                   const std::vector<const std::vector<double> *> &progs, bool keyed_pc,
+                   CodeSet &cs, std::vector<std::vector<int32_t>> &recs, std::string &err) {
     const size_t nl = entries.size();
     // distinct records of each line and their element counts
     std::vector<std::map<int64_t, int64_t>> uses(nl);
@@ -228,7 +228,7 @@ int build_code_set(const std::vector<const std::vector<int64_t> *> &entries,
         for (int64_t e : *entries[l]) uses[l][e]++;
         n_distinct += (int64_t)uses[l].size();
     }
-    const bool all_literal = nl == 1 && n_distinct <= kLiteralAll;
+    const bool all_literal = !keyed_pc && nl == 1 && n_distinct <= kLiteralAll;
     cs.keyed_pc = !all_literal;
     // literal text of every record (program addresses relative to the
     // record unless every record of a single line is literal)
@@ -451,7 +451,7 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
                             std::vector<int32_t> &rec_of_elem, std::string &err) {
     CodeSet cs;
     std::vector<std::vector<int32_t>> recs;
-    const int st = build_code_set({&entry}, {&prog}, cs, recs, err);
+    const int st = build_code_set({&entry}, {&prog}, false, cs, recs, err);
     if (st != XTM_OK) return st;
     rec_of_elem = std::move(recs[0]);
     std::string s = emit_code_set(cs, options, "line code");
@@ -467,14 +467,14 @@ int xt_generate_line_source(const std::vector<int64_t> &entry, const std::vector
          "    int32_t num_turns, int32_t ele_start, int32_t num_ele_track,\n"
          "    int32_t flag_end_turn_actions, int32_t flag_reset_s, int32_t flag_monitor,\n"
          "    const xtm_monitor *__restrict__ Md, const xtm_particles *__restrict__ Pd,\n"
-         "    const xtm_features *__restrict__ Fd, uint64_t flags) {\n"
+         "    uint64_t flags) {\n"
          "    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
          "    if (i >= P.capacity) return;\n"
          "    const xt::TrackCtx c{flags, line_length, P.q0, P.t_sim};\n"
          "    // chi == 1 particles; the interpreter kernel tracks the others\n"
          "    xt::track_particle_chi1(P, i, xt_gen::Runner{rec, entry, prog}, c, num_turns,\n"
          "                            ele_start, num_ele_track, flag_end_turn_actions,\n"
-         "                            flag_reset_s, flag_monitor, xt::MonRef{Md, Pd, Fd});\n"
+         "                            flag_reset_s, flag_monitor, xt::MonRef{Md, Pd});\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over all particles\n"
@@ -502,12 +502,9 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
                              int32_t waves_per_eu, std::string &src,
                              std::vector<std::vector<int32_t>> &recs, std::string &err) {
     CodeSet cs;
-    const int st = build_code_set(entries, progs, cs, recs, err);
+    // the batch kernel's runner reads each element's program offset
+    const int st = build_code_set(entries, progs, true, cs, recs, err);
     if (st != XTM_OK) return st;
-    if (!cs.keyed_pc) {
-        err = "internal: batch code without program offsets";
-        return XTM_ERR_ARG;
-    }
     std::string s = emit_code_set(cs, 0, "batch code");
     s += "/* one block of a batch: particles [first, first + count) of line `line` */\n"
          "struct BatchBlock { int64_t first; int32_t line; int32_t count; };\n"
@@ -522,7 +519,7 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
          "    const int32_t *const *__restrict__ recs, const int64_t *const *__restrict__ entries,\n"
          "    const double *const *__restrict__ progs, const int64_t *__restrict__ nele,\n"
          "    const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,\n"
-         "    const xtm_features *__restrict__ Fd, int32_t num_turns, uint64_t flags) {\n"
+         "    int32_t num_turns, uint64_t flags) {\n"
          "    const xt_gen::BatchBlock b = xt::cptr(blocks)[blockIdx.x];\n"
          "    if ((int32_t)threadIdx.x >= b.count) return;\n"
          "    const xtm_particles &P = *Pd;\n"
@@ -533,7 +530,7 @@ int xt_generate_batch_source(const std::vector<const std::vector<int64_t> *> &en
          "    xt::track_particle_chi1(P, i, xt_gen::Runner{xt::cptr(recs)[b.line],\n"
          "                            xt::cptr(entries)[b.line], xt::cptr(progs)[b.line]}, c,\n"
          "                            num_turns, 0, (int32_t)ne, 1, 1, 0,\n"
-         "                            xt::MonRef{nullptr, Pd, Fd});\n"
+         "                            xt::MonRef{nullptr, Pd});\n"
          "}\n"
          "#else\n"
          "// host build (tests): the same per-particle body over the blocks\n"

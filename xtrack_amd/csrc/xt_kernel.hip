@@ -63,7 +63,7 @@ __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     int64_t num_ele_line, const xtm_particles *__restrict__ Pd, int32_t num_turns,
     int32_t ele_start, int32_t num_ele_track, int32_t flag_end_turn_actions,
     int32_t flag_reset_s, int32_t flag_monitor, const xtm_monitor *__restrict__ Md,
-    const xtm_features *__restrict__ Fd, uint64_t flags, int32_t skip_chi1) {
+    uint64_t flags, int32_t skip_chi1) {
     const xtm_particles &P = *Pd;
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= P.capacity) return;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) XTM_OCCUPANCY void track_kernel(
     if (skip_chi1 && P.chi[i] == 1.0) return;
     xt::track_particle(P, i, entry, prog, line_length, num_ele_line, num_turns, ele_start,
                        num_ele_track, flag_end_turn_actions, flag_reset_s, flag_monitor,
-                       xt::MonRef{Md, Pd, Fd}, flags);
+                       xt::MonRef{Md, Pd}, flags);
 }
 
 /* One block of a batch: particles [first, first + count) of line `line`. */
@@ -90,8 +90,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const BatchBlock *__restrict__ blocks, const int64_t *const *__restrict__ entries,
     const double *const *__restrict__ progs, const int64_t *__restrict__ nele,
     const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,
-    const xtm_features *__restrict__ Fd, int32_t num_turns, uint64_t flags,
-    int32_t skip_chi1) {
+    int32_t num_turns, uint64_t flags, int32_t skip_chi1) {
     static_assert(BS <= BLOCK, "the per-thread LDS slots are laid out for 256 threads");
     const BatchBlock b = xt::cptr(blocks)[blockIdx.x];
     if ((int32_t)threadIdx.x >= b.count) return;
@@ -103,20 +102,28 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const xt::TrackCtx c{flags, xt::cptr(lengths)[b.line], P.q0, P.t_sim, ne};
     const xt::ProgramRunner run{xt::cptr(entries)[b.line], xt::cptr(progs)[b.line]};
     xt::track_particle_with<false>(P, i, run, run, c, num_turns, 0, (int32_t)ne, 1, 1, 0,
-                                   xt::MonRef{nullptr, Pd, Fd});
+                                   xt::MonRef{nullptr, Pd});
 }
 
-/* device copies of the particle struct, the monitor and the feature
-   accumulators' description for the tracking kernels (stream-ordered before
-   them; kernel arguments are captured at launch, so the caller's structs
-   need not outlive the call) */
-__global__ void args_kernel(xtm_particles P, xtm_monitor M, xtm_features F, xtm_particles *Pd,
-                            xtm_monitor *Md, xtm_features *Fd) {
+/* device copies of the particle struct and the monitor for the tracking
+   kernels (stream-ordered before them; kernel arguments are captured at
+   launch, so the caller's structs need not outlive the call) */
+__global__ void args_kernel(xtm_particles P, xtm_monitor M, xtm_particles *Pd,
+                            xtm_monitor *Md) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *Pd = P;
         *Md = M;
-        if (Fd) *Fd = F;
     }
+}
+
+/* Feature accumulators of every active particle at the start of its
+   current turn (the state the last tracking launch left), one thread per
+   particle slot: runs between one-turn tracking launches, so the tracking
+   kernels carry no trace of it. */
+__global__ __launch_bounds__(256) void features_kernel(xtm_features F, xtm_particles P) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.capacity || !(P.state[i] > 0)) return;
+    xt::feature_record(F, i, P.x[i], P.px[i], P.y[i], P.py[i], P.at_turn[i]);
 }
 
 __global__ void math_kernel(int32_t fn, const double *__restrict__ x, double *__restrict__ y,
@@ -171,7 +178,6 @@ struct xtm_line {
     // records (written by monitor_args_kernel before a launch with a monitor)
     xtm_particles *d_mon_p = nullptr;
     xtm_monitor *d_mon_m = nullptr;
-    xtm_features *d_feat = nullptr;
     // generated line code (xtm_line_specialize)
     int32_t *d_rec = nullptr;
     hipModule_t module = nullptr;
@@ -195,7 +201,6 @@ struct xtm_batch {
     int64_t d_blocks_cap = 0;
     xtm_particles *d_p = nullptr;
     xtm_monitor *d_m = nullptr;
-    xtm_features *d_f = nullptr;
     // generated batch code (xtm_batch_specialize)
     std::vector<int32_t *> d_recs;
     const int32_t **d_rec_ptrs = nullptr;
@@ -318,8 +323,7 @@ int32_t xtm_line_create(const int32_t *types, const int64_t *offsets, const doub
         (e = hipMalloc(&L->d_entry_bt, ne * sizeof(int64_t))) != hipSuccess ||
         (e = hipMalloc(&L->d_prog, prog.size() * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&L->d_mon_p, sizeof(xtm_particles))) != hipSuccess ||
-        (e = hipMalloc(&L->d_mon_m, sizeof(xtm_monitor))) != hipSuccess ||
-        (e = hipMalloc(&L->d_feat, sizeof(xtm_features))) != hipSuccess) {
+        (e = hipMalloc(&L->d_mon_m, sizeof(xtm_monitor))) != hipSuccess) {
         xtm_line_destroy(L);
         return hip_fail(e, "hipMalloc");
     }
@@ -352,7 +356,6 @@ int32_t xtm_line_destroy(xtm_line *L) {
     if (L->d_rec) (void)hipFree(L->d_rec);
     if (L->d_mon_p) (void)hipFree(L->d_mon_p);
     if (L->d_mon_m) (void)hipFree(L->d_mon_m);
-    if (L->d_feat) (void)hipFree(L->d_feat);
     if (L->module) (void)hipModuleUnload(L->module);
     delete L;
     return XTM_OK;
@@ -467,8 +470,8 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
                                int32_t ele_start, int32_t num_ele_track,
                                int32_t flag_end_turn_actions,
                                int32_t flag_reset_s_at_end_turn, int32_t flag_monitor,
-                               const xtm_monitor *monitor, const xtm_features *features,
-                               uint64_t track_flags, void *stream) {
+                               const xtm_monitor *monitor, uint64_t track_flags,
+                               void *stream) {
     if (!L || !P) return fail(XTM_ERR_ARG, "xtm_track_line: null line or particles");
     if (num_turns < 0 || ele_start < 0 || num_ele_track < 0 ||
         (int64_t)ele_start + num_ele_track > L->num_elements)
@@ -481,21 +484,14 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
         return fail(XTM_ERR_UNSUPPORTED, "SR tapering is not supported");
     if (P->capacity < 0) return fail(XTM_ERR_ARG, "xtm_track_line: negative capacity");
     if (L->device < 0) return fail(XTM_ERR_ARG, "xtm_track_line: line was created without a device");
-    if (!features_ok(features, P->capacity))
-        return fail(XTM_ERR_ARG, "xtm_track_line_features: accumulators missing or of the "
-                                 "wrong capacity");
     if (P->capacity == 0 || num_turns == 0) return XTM_OK;
     hipError_t e = hipSetDevice(L->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     xtm_monitor M;
     if (flag_monitor) M = *monitor;
     else std::memset(&M, 0, sizeof(M));
-    xtm_features F;
-    if (features) F = *features;
-    else std::memset(&F, 0, sizeof(F));
-    const xtm_features *Fd = features ? L->d_feat : nullptr;
-    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, *P, M, F,
-                       L->d_mon_p, L->d_mon_m, L->d_feat);
+    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, *P, M,
+                       L->d_mon_p, L->d_mon_m);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "args_kernel launch");
     const xtm_monitor *Md = L->d_mon_m;
@@ -514,7 +510,7 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
         int32_t a_mon = flag_monitor;
         uint64_t a_flags = track_flags;
         void *args[] = {&rec, &entry, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
-                        &a_end, &a_reset, &a_mon, &Md, &Pd, &Fd, &a_flags};
+                        &a_end, &a_reset, &a_mon, &Md, &Pd, &a_flags};
         e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
                                   (hipStream_t)stream, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
@@ -523,7 +519,7 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
                            (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length,
                            (int64_t)L->num_elements, Pd, num_turns, ele_start, num_ele_track,
                            flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor, Md,
-                           Fd, track_flags, (int32_t)1);
+                           track_flags, (int32_t)1);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
         return XTM_OK;
@@ -534,7 +530,7 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
                        (hipStream_t)stream, backtrack ? L->d_entry_bt : L->d_entry, L->d_prog,
                        L->line_length, (int64_t)L->num_elements, Pd,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
-                       flag_reset_s_at_end_turn, flag_monitor, Md, Fd, track_flags, (int32_t)0);
+                       flag_reset_s_at_end_turn, flag_monitor, Md, track_flags, (int32_t)0);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "track_kernel launch");
     return XTM_OK;
@@ -545,8 +541,22 @@ int32_t xtm_track_line(xtm_line *L, const xtm_particles *P, int32_t num_turns,
                        int32_t flag_reset_s_at_end_turn, int32_t flag_monitor,
                        const xtm_monitor *monitor, uint64_t track_flags, void *stream) {
     return track_line_impl(L, P, num_turns, ele_start, num_ele_track, flag_end_turn_actions,
-                           flag_reset_s_at_end_turn, flag_monitor, monitor, nullptr,
-                           track_flags, stream);
+                           flag_reset_s_at_end_turn, flag_monitor, monitor, track_flags,
+                           stream);
+}
+
+int32_t xtm_features_record(const xtm_particles *P, const xtm_features *F, void *stream) {
+    if (!P || !F) return fail(XTM_ERR_ARG, "xtm_features_record: null argument");
+    if (!features_ok(F, P->capacity))
+        return fail(XTM_ERR_ARG, "xtm_features_record: accumulators missing or of the wrong "
+                                 "capacity");
+    if (P->capacity == 0) return XTM_OK;
+    const int64_t nblocks = (P->capacity + 255) / 256;
+    if (nblocks > 0x7fffffff) return fail(XTM_ERR_ARG, "too many particles");
+    hipLaunchKernelGGL(features_kernel, dim3((unsigned)nblocks), dim3(256), 0,
+                       (hipStream_t)stream, *F, *P);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? XTM_OK : hip_fail(e, "features_kernel launch");
 }
 
 int32_t xtm_track_line_features(xtm_line *L, const xtm_particles *P, int32_t num_turns,
@@ -556,8 +566,14 @@ int32_t xtm_track_line_features(xtm_line *L, const xtm_particles *P, int32_t num
         return fail(XTM_ERR_ARG, "xtm_track_line_features: null argument");
     if ((track_flags >> XTM_FLAG_BACKTRACK) & 1)
         return fail(XTM_ERR_UNSUPPORTED, "xtm_track_line_features: tracks forward only");
-    return track_line_impl(L, P, num_turns, 0, (int32_t)L->num_elements, 1, 1, 0, nullptr,
-                           features, track_flags, stream);
+    for (int32_t t = 0; t < num_turns; t++) {
+        int32_t st = xtm_features_record(P, features, stream);
+        if (st == XTM_OK)
+            st = track_line_impl(L, P, 1, 0, (int32_t)L->num_elements, 1, 1, 0, nullptr,
+                                 track_flags, stream);
+        if (st != XTM_OK) return st;
+    }
+    return XTM_OK;
 }
 
 
@@ -580,7 +596,6 @@ int32_t xtm_batch_destroy(xtm_batch *B) {
         if (B->d_blocks) (void)hipFree(B->d_blocks);
         if (B->d_p) (void)hipFree(B->d_p);
         if (B->d_m) (void)hipFree(B->d_m);
-        if (B->d_f) (void)hipFree(B->d_f);
         for (int32_t *r : B->d_recs)
             if (r) (void)hipFree(r);
         if (B->d_rec_ptrs) (void)hipFree(B->d_rec_ptrs);
@@ -629,8 +644,7 @@ int32_t xtm_batch_create(xtm_line *const *lines, int32_t num_lines, xtm_batch **
         (e = hipMalloc(&B->d_nele, n * sizeof(int64_t))) != hipSuccess ||
         (e = hipMalloc(&B->d_lengths, n * sizeof(double))) != hipSuccess ||
         (e = hipMalloc(&B->d_p, sizeof(xtm_particles))) != hipSuccess ||
-        (e = hipMalloc(&B->d_m, sizeof(xtm_monitor))) != hipSuccess ||
-        (e = hipMalloc(&B->d_f, sizeof(xtm_features))) != hipSuccess) {
+        (e = hipMalloc(&B->d_m, sizeof(xtm_monitor))) != hipSuccess) {
         xtm_batch_destroy(B);
         return hip_fail(e, "hipMalloc");
     }
@@ -709,8 +723,7 @@ int32_t xtm_batch_copy_blocks(const xtm_batch *B, int64_t *out) {
 }
 
 static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t num_turns,
-                                const xtm_features *features, uint64_t track_flags,
-                                void *stream) {
+                                uint64_t track_flags, void *stream) {
     if (!B || !P) return fail(XTM_ERR_ARG, "xtm_batch_track: null batch or particles");
     if (B->device < 0) return fail(XTM_ERR_ARG, "xtm_batch_track: batch was created without a device");
     if (B->line_start.empty()) return fail(XTM_ERR_ARG, "xtm_batch_track: no layout set");
@@ -721,20 +734,14 @@ static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t nu
         return fail(XTM_ERR_UNSUPPORTED, "SR tapering is not supported");
     if ((track_flags >> XTM_FLAG_BACKTRACK) & 1)
         return fail(XTM_ERR_UNSUPPORTED, "xtm_batch_track: backtracking is not supported");
-    if (!features_ok(features, P->capacity))
-        return fail(XTM_ERR_ARG, "xtm_batch_track_features: accumulators missing or of the "
-                                 "wrong capacity");
+
     if (num_turns == 0 || B->blocks.empty()) return XTM_OK;
     hipError_t e = hipSetDevice(B->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     xtm_monitor M;
     std::memset(&M, 0, sizeof(M));
     const hipStream_t s = (hipStream_t)stream;
-    xtm_features F;
-    if (features) F = *features;
-    else std::memset(&F, 0, sizeof(F));
-    const xtm_features *Fd = features ? B->d_f : nullptr;
-    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, s, *P, M, F, B->d_p, B->d_m, B->d_f);
+    hipLaunchKernelGGL(args_kernel, dim3(1), dim3(64), 0, s, *P, M, B->d_p, B->d_m);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "args_kernel launch");
     const dim3 grid((unsigned)B->blocks.size());
@@ -749,7 +756,7 @@ static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t nu
         const xtm_particles *Pd = B->d_p;
         int32_t a_turns = num_turns;
         uint64_t a_flags = track_flags;
-        void *args[] = {&blocks, &recs, &entries, &progs, &nele, &lengths, &Pd, &Fd, &a_turns,
+        void *args[] = {&blocks, &recs, &entries, &progs, &nele, &lengths, &Pd, &a_turns,
                         &a_flags};
         e = hipModuleLaunchKernel(B->kernel, grid.x, 1, 1, BLOCK, 1, 1, 0, s, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_batch_line_kernel launch");
@@ -757,13 +764,13 @@ static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t nu
     }
     if (B->waves_per_eu == 5)
         hipLaunchKernelGGL((batch_kernel<256, 5>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, Fd, num_turns, track_flags, skip);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else if (B->waves_per_eu == 4)
         hipLaunchKernelGGL((batch_kernel<256, 4>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, Fd, num_turns, track_flags, skip);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else
         hipLaunchKernelGGL((batch_kernel<256, 3>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, Fd, num_turns, track_flags, skip);
+                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch_kernel launch");
     return XTM_OK;
@@ -771,14 +778,20 @@ static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t nu
 
 int32_t xtm_batch_track(xtm_batch *B, const xtm_particles *P, int32_t num_turns,
                         uint64_t track_flags, void *stream) {
-    return batch_track_impl(B, P, num_turns, nullptr, track_flags, stream);
+    return batch_track_impl(B, P, num_turns, track_flags, stream);
 }
 
 int32_t xtm_batch_track_features(xtm_batch *B, const xtm_particles *P, int32_t num_turns,
                                  const xtm_features *features, uint64_t track_flags,
                                  void *stream) {
-    if (!features) return fail(XTM_ERR_ARG, "xtm_batch_track_features: null accumulators");
-    return batch_track_impl(B, P, num_turns, features, track_flags, stream);
+    if (!B || !P || !features)
+        return fail(XTM_ERR_ARG, "xtm_batch_track_features: null argument");
+    for (int32_t t = 0; t < num_turns; t++) {
+        int32_t st = xtm_features_record(P, features, stream);
+        if (st == XTM_OK) st = batch_track_impl(B, P, 1, track_flags, stream);
+        if (st != XTM_OK) return st;
+    }
+    return XTM_OK;
 }
 
 int32_t xtm_batch_generate_source(const xtm_batch *B, int32_t waves_per_eu, char *buf,

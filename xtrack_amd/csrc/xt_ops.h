@@ -1306,13 +1306,11 @@ XT_HD_COLD void monitor_record(const xtm_monitor &M, const xtm_particles &P, int
 struct MonRef {
     const xtm_monitor *M;
     const xtm_particles *P;
-    const xtm_features *F = nullptr;   // feature accumulators, or none
 };
 
 /* Feature accumulators of particle slot i at the start of a turn (see
-   xtm_features); out of line with scalar arguments, like the monitor
-   record, so the element loop carries none of its registers. */
-XT_HD_COLD void feature_record(const xtm_features &F, const int64_t i, const double x,
+   xtm_features), from its coordinates between two tracking launches. */
+XT_HD void feature_record(const xtm_features &F, const int64_t i, const double x,
                                const double px, const double y, const double py,
                                const int64_t at_turn) {
     if (at_turn < F.start_at_turn || at_turn >= F.stop_at_turn) return;
@@ -1413,7 +1411,6 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
         int64_t base = P.at_element[i] - ele_start;
         for (int32_t iturn = 0; iturn < num_turns; iturn++) {
             if (flag_monitor == 1) XT_MONITOR_RECORD(base + ele_start);
-            if (mon.F) feature_record(*mon.F, i, p.x, p.px, p.y, p.py, p.at_turn);
             /* the next element's key (record index / program offset) is
                loaded one element ahead, so its scalar load overlaps the
                current element's arithmetic */

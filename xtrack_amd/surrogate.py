@@ -171,7 +171,8 @@ class MLP:
 
 
 # ---------------------------------------------------------------------------
-# dynamic-aperture scan screened by the surrogate
+# dynamic-aperture scans over one or many lattices (error seeds), on the batch
+# tracker, plain and screened by the surrogate
 
 def _take(p, idx):
     """Particles holding the slots `idx` (device index tensor) of `p`."""
@@ -192,10 +193,21 @@ def _take(p, idx):
     return out
 
 
+def _put(p, idx, sub):
+    """Write the particles of `sub` back into slots `idx` of `p`."""
+    torch = _t()
+    for nn, vv in p._arrays.items():
+        if vv.dtype == torch.uint32:
+            vv.view(torch.int32)[idx] = sub._arrays[nn].view(torch.int32)
+        else:
+            vv[idx] = sub._arrays[nn]
+
+
 def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, starts=None):
     """Track `particles` (a device particle set) for num_turns whole turns
-    with the feature accumulators updated inside the tracking kernel at the
-    start of every turn (xtm_track_line_features / xtm_batch_track_features);
+    with the feature accumulators updated on the device at the start of every
+    turn, between one-turn tracking launches (xtm_track_line_features /
+    xtm_batch_track_features);
     returns the accumulators, a (XTM_FEAT_N, capacity) float64 tensor on the
     device, row k = accumulator k of every particle slot.
 
@@ -253,162 +265,313 @@ def features_from_acc(acc, x_norm0, y_norm0, delta):
     return torch.nan_to_num(f, nan=0.0, posinf=50.0, neginf=-50.0).to(torch.float32)
 
 
-def features(mon, x_norm0, y_norm0, delta, invW4, sx, sy, n_turns):
-    """16 features per particle from its initial normalised position and
-    the turn-by-turn record of its first n_turns turns: r, theta, delta,
-    the initial actions, mean, spread and maximum of the normalised actions
-    over the turns (relative to the initial ones), the mean phase advance
-    per turn in each plane (tunes), the fraction of the turns survived."""
+class GridSet:
+    """The grid particles of every lattice in one device particle set:
+    lattice l holds slots starts[l]..starts[l+1], in grid order (r, theta,
+    delta as da.polar_grid flattens them), particle_id = slot."""
+
+    def __init__(self, lines, r_max, n_r, n_angles, deltas, nemitt_x, nemitt_y, device):
+        from . import da
+        from .particles import Particles
+        from .twiss import _reference
+        self.r, self.theta, self.delta = da.polar_grid(r_max, n_r, n_angles, deltas)
+        self.n_grid = len(self.r)
+        self.n_lines = len(lines)
+        xn, yn = self.r * np.cos(self.theta), self.r * np.sin(self.theta)
+        hosts = []
+        for ln in lines:
+            pp = da.build_normalized_particles(ln, xn, 0.0, yn, 0.0, self.delta, nemitt_x,
+                                               nemitt_y)
+            hosts.append(pp.to_numpy())
+        p0 = _reference(lines[0], None)
+        big = Particles.__new__(Particles)
+        big._capacity = self.n_grid * self.n_lines
+        big.q0, big.mass0, big.t_sim = p0.q0, p0.mass0, 0.0
+        big.start_tracking_at_element = -1
+        big._arrays = {nn: np.concatenate([h[nn] for h in hosts]) for nn in hosts[0]}
+        big._arrays['particle_id'] = np.arange(big._capacity, dtype=np.int64)
+        big._device = None
+        big.move(_context=device)
+        self.particles = big
+        self._initial = {nn: vv.clone() for nn, vv in big._arrays.items()}
+        self.starts = np.arange(self.n_lines + 1, dtype=np.int64) * self.n_grid
+        self.x_norm = np.tile(xn, self.n_lines)
+        self.y_norm = np.tile(yn, self.n_lines)
+        self.delta_all = np.tile(self.delta, self.n_lines)
+        bg = float(p0._np('beta0')[0] * p0._np('gamma0')[0])
+        self.sigma = (math.sqrt(nemitt_x / bg), math.sqrt(nemitt_y / bg))
+        self.device = device
+
+    def line_of(self, slots):
+        return np.searchsorted(self.starts, slots, side='right') - 1
+
+    def reset(self):
+        """Back to the initial grid particles (for another scan)."""
+        for nn, vv in self.particles._arrays.items():
+            vv.copy_(self._initial[nn])
+
+
+def _starts_of(slots_sorted, starts):
+    """Layout of a sorted subset of slots: the start of each lattice's
+    range within the subset."""
+    return np.searchsorted(slots_sorted, starts).astype(np.int64)
+
+
+def track_subset(bt, gs, slots, num_turns, block=None, features=None):
+    """Track the grid particles `slots` (sorted numpy slot indices) of
+    GridSet gs for num_turns turns on BatchTracker bt, compacting the lost
+    ones out every `block` turns (all turns in one launch when None), and
+    write the results back into gs.particles.  features: None, or
+    (inv_w4, co) to gather the feature accumulators (with block None);
+    returns them for the slots, else None."""
     torch = _t()
-    X = torch.stack([mon.x, mon.px, mon.y, mon.py])            # 4 x N x T
-    st = mon.state
-    alive = (st > 0).to(torch.float64)
-    Wi = torch.as_tensor(invW4, dtype=torch.float64, device=X.device)
-    Z = torch.einsum('ij,jnt->int', Wi, X.to(torch.float64))
-    zx, zpx = Z[0] / sx, Z[1] / sx
-    zy, zpy = Z[2] / sy, Z[3] / sy
-    jx, jy = 0.5 * (zx * zx + zpx * zpx), 0.5 * (zy * zy + zpy * zpy)
-    nal = alive.sum(dim=1).clamp_min(1.0)
-    r0 = torch.sqrt(torch.as_tensor(x_norm0 ** 2 + y_norm0 ** 2, device=X.device))
-    th0 = torch.atan2(torch.as_tensor(y_norm0, device=X.device),
-                      torch.as_tensor(x_norm0, device=X.device))
-    j0x, j0y = jx[:, 0].clamp_min(1e-12), jy[:, 0].clamp_min(1e-12)
-    mjx = (jx * alive).sum(1) / nal
-    mjy = (jy * alive).sum(1) / nal
-    sjx = torch.sqrt(((jx - mjx[:, None]) ** 2 * alive).sum(1) / nal)
-    sjy = torch.sqrt(((jy - mjy[:, None]) ** 2 * alive).sum(1) / nal)
-    mxx = (jx * alive).amax(1)
-    mxy = (jy * alive).amax(1)
+    dev = gs.device
+    slots = np.asarray(slots, dtype=np.int64)
+    done, acc = 0, None
+    while done < num_turns and len(slots):
+        n = num_turns - done if block is None else min(block, num_turns - done)
+        idx = torch.as_tensor(slots, device=dev)
+        sub = _take(gs.particles, idx)
+        lay = _starts_of(slots, gs.starts)
+        if features is not None:
+            inv_w4, co = features
+            acc = track_features(bt, sub, n, inv_w4, co, gs.sigma[0], gs.sigma[1], starts=lay)
+        else:
+            bt._set_layout(lay, bt.config or (256, 4))
+            bt._launch(sub, n)
+        _put(gs.particles, idx, sub)
+        done += n
+        if done < num_turns:
+            alive = (sub._arrays['state'] > 0).cpu().numpy()
+            slots = slots[alive]
+    torch.cuda.synchronize(dev)
+    return acc
 
-    def tune(z, pz):
-        ph = torch.atan2(-pz, z)
-        d = torch.remainder(ph[:, 1:] - ph[:, :-1], 2 * math.pi)
-        a = alive[:, 1:]
-        return (d * a).sum(1) / a.sum(1).clamp_min(1.0) / (2 * math.pi)
 
-    f = torch.stack([r0, th0, torch.as_tensor(delta * 1e3, device=X.device),
-                     torch.log1p(j0x), torch.log1p(j0y),
-                     torch.log(mjx.clamp_min(1e-12) / j0x), torch.log(mjy.clamp_min(1e-12) / j0y),
-                     sjx / mjx.clamp_min(1e-12), sjy / mjy.clamp_min(1e-12),
-                     torch.log(mxx.clamp_min(1e-12) / j0x), torch.log(mxy.clamp_min(1e-12) / j0y),
-                     tune(zx, zpx), tune(zy, zpy), nal / n_turns,
-                     torch.cos(th0), torch.sin(th0)], dim=1)
-    return torch.nan_to_num(f, nan=0.0, posinf=50.0, neginf=-50.0).to(torch.float32)
+def survival_of(gs, num_turns):
+    """Per slot: the turn of loss, or num_turns for a particle alive."""
+    st = gs.particles._arrays['state'].cpu().numpy()
+    at = gs.particles._arrays['at_turn'].cpu().numpy()
+    return np.where(st > 0, num_turns, at).astype(np.int64)
+
+
+def da_per_line(gs, surv, num_turns):
+    """[{(theta, delta): DA}] per lattice (da.dynamic_aperture)."""
+    from . import da
+    out = []
+    for l in range(gs.n_lines):
+        sl = slice(gs.starts[l], gs.starts[l + 1])
+        out.append(da.dynamic_aperture(gs.r, gs.theta, gs.delta, surv[sl], num_turns))
+    return out
+
+
+def plain_da_scan_batch(bt, gs, num_turns, block=256):
+    """Every grid particle of every lattice tracked to num_turns (lost
+    particles compacted out every `block` turns).  Returns survival turns,
+    the apertures per lattice and the wall time."""
+    torch = _t()
+    torch.cuda.synchronize(gs.device)
+    t0 = time.perf_counter()
+    track_subset(bt, gs, np.arange(gs.n_grid * gs.n_lines), num_turns, block=block)
+    wall = time.perf_counter() - t0
+    surv = survival_of(gs, num_turns)
+    return {'survival': surv, 'da': da_per_line(gs, surv, num_turns), 'time': wall}
+
+
+def _group_keys(gs, slots):
+    """(lattice, angle index, delta index) of each slot."""
+    g = slots % gs.n_grid
+    th_idx = np.searchsorted(np.unique(gs.theta), gs.theta[g])
+    d_idx = np.searchsorted(np.unique(gs.delta), gs.delta[g])
+    return gs.line_of(slots), th_idx, d_idx
+
+
+def interp_predictor(gs, labelled, surv, candidates):
+    """Survival prediction without a network, for comparison: within each
+    (lattice, angle, delta) group, log survival interpolated in amplitude
+    between the labelled particles of the group (their end values beyond
+    them; 1 turn for a group without labels)."""
+    lab = np.asarray(labelled, dtype=np.int64)
+    cand = np.asarray(candidates, dtype=np.int64)
+    pred = np.ones(len(cand))
+    lk = np.stack(_group_keys(gs, lab), axis=1) if len(lab) else np.zeros((0, 3), np.int64)
+    ck = np.stack(_group_keys(gs, cand), axis=1)
+    groups = {}
+    for i, k in enumerate(map(tuple, lk)):
+        groups.setdefault(k, []).append(lab[i])
+    by_key = {}
+    for i, k in enumerate(map(tuple, ck)):
+        by_key.setdefault(k, []).append(i)
+    for k, members in by_key.items():
+        g = groups.get(k)
+        if not g:
+            continue
+        g = np.asarray(g)
+        g = g[np.argsort(gs.r[g % gs.n_grid])]
+        m = np.asarray(members)
+        pred[m] = np.exp(np.interp(gs.r[cand[m] % gs.n_grid], gs.r[g % gs.n_grid],
+                                   np.log(np.maximum(surv[g], 1))))
+    return pred
+
+
+def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma):
+    """The slots of `rest` to track in full: guessed survival (exact where
+    known, predicted for `rest`) within a factor band_factor of num_turns,
+    or amplitude within band_sigma of the first guessed loss of the
+    particle's (lattice, angle, delta) group (the predicted aperture; the
+    largest amplitude when no loss is guessed)."""
+    rest = np.asarray(rest, dtype=np.int64)
+    pred = guess[rest]
+    in_band = (pred >= num_turns / band_factor) & (pred <= num_turns * band_factor)
+    n_all = len(guess)
+    r_all = gs.r[np.arange(n_all) % gs.n_grid]
+    keys = np.stack(_group_keys(gs, np.arange(n_all)), axis=1)
+    gid = np.unique(keys, axis=0, return_inverse=True)[1].reshape(-1)
+    lost_r = np.full(int(gid.max()) + 1, np.inf)
+    lost = guess < num_turns
+    np.minimum.at(lost_r, gid[lost], r_all[lost])
+    r_pred = np.where(np.isfinite(lost_r), lost_r, gs.r.max())
+    in_band |= np.abs(r_all[rest] - r_pred[gid[rest]]) <= band_sigma
+    return rest[in_band]
+
+
+def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
+                     band_factor=3.0, band_sigma=0.5, epochs=150, seed=0, net=None,
+                     predictor='mlp', block=256, normalisation=None):
+    """Dynamic-aperture scan of every lattice of GridSet gs screened by the
+    survival surrogate, on BatchTracker bt (the lattices of gs, in order):
+
+    1. every grid particle is tracked for feature_turns turns with the
+       feature accumulators gathered in the kernel;
+    2. a random label_fraction of the survivors of each lattice is tracked
+       to num_turns: their survival turns are the labels;
+    3. the network (16-256-256-256-1, BF16 products on the matrix cores) is
+       fit to log survival turn on the labels (predictor='interp': no
+       network, per-angle interpolation of the labels in amplitude);
+    4. the survivors whose predicted survival lies within a factor
+       band_factor of num_turns, or whose amplitude lies within band_sigma
+       beam sigmas of their angle's predicted aperture, are tracked to
+       num_turns (the boundary band);
+    5. every other particle keeps its prediction: alive if predicted to
+       survive num_turns, else lost at its predicted turn.
+
+    Tracking is continued, not restarted: a particle tracked in full has
+    exactly its plain-scan survival turn.  normalisation: (inv_w4, co) of
+    the features, default the first lattice's 4d normal form at delta 0."""
+    torch = _t()
+    dev = gs.device
+    t_all = time.perf_counter()
+    n_all = gs.n_grid * gs.n_lines
+    if normalisation is None:
+        normalisation = normal_form_4d(bt.lines[0], dev)
+    times = {}
+    # 1. feature turns
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    acc = track_subset(bt, gs, np.arange(n_all), feature_turns, features=normalisation)
+    feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_all)
+    torch.cuda.synchronize(dev)
+    times['features'] = time.perf_counter() - t0
+    surv = survival_of(gs, feature_turns)
+    exact = surv < feature_turns
+    alive = np.flatnonzero(~exact)
+    # 2. labels
+    rng = np.random.default_rng(seed)
+    lab = []
+    for l in range(gs.n_lines):
+        al = alive[(alive >= gs.starts[l]) & (alive < gs.starts[l + 1])]
+        k = min(len(al), max(16, int(round(label_fraction * len(al)))))
+        if k:
+            lab.append(np.sort(rng.choice(al, size=k, replace=False)))
+    lab = np.concatenate(lab) if lab else np.zeros(0, np.int64)
+    t0 = time.perf_counter()
+    track_subset(bt, gs, lab, num_turns - feature_turns, block=block)
+    times['label_tracking'] = time.perf_counter() - t0
+    surv = survival_of(gs, num_turns)
+    exact[lab] = True
+    # 3. predictor
+    t0 = time.perf_counter()
+    rest = np.flatnonzero(~exact)
+    hist = None
+    if predictor == 'mlp':
+        net = net or MLP(device=dev, seed=seed)
+        fx = feat[torch.as_tensor(lab, device=dev)]
+        fy = torch.as_tensor(np.log(surv[lab].astype(np.float64)), dtype=torch.float32,
+                             device=dev)
+        hist = net.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(lab) // 4)),
+                       seed=seed)
+        pred = np.exp(np.minimum(net.predict(feat[torch.as_tensor(rest, device=dev)])
+                                 .cpu().numpy().astype(np.float64), math.log(1e12)))
+    elif predictor == 'interp':
+        pred = interp_predictor(gs, lab, surv, rest)
+    else:
+        raise ValueError("predictor must be 'mlp' or 'interp'")
+    torch.cuda.synchronize(dev)
+    times['fit_predict'] = time.perf_counter() - t0
+    # 4. boundary band: predicted survival near num_turns, or amplitude near
+    # the predicted aperture of its (lattice, angle, delta) group
+    guess = surv.astype(np.float64)
+    guess[rest] = pred
+    band = boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma)
+    t0 = time.perf_counter()
+    track_subset(bt, gs, band, num_turns - feature_turns, block=block)
+    times['band_tracking'] = time.perf_counter() - t0
+    surv_t = survival_of(gs, num_turns)
+    surv[band] = surv_t[band]
+    exact[band] = True
+    # 5. the rest keeps its prediction
+    left = np.flatnonzero(~exact)
+    p_left = guess[left]
+    surv[left] = np.where(p_left >= num_turns, num_turns,
+                          np.clip(p_left, feature_turns, num_turns - 1)).astype(np.int64)
+    times['total'] = time.perf_counter() - t_all
+    return {'survival': surv, 'exact': exact, 'labelled': lab, 'band': band,
+            'tracked_full_fraction': (len(lab) + len(band)) / n_all,
+            'da': da_per_line(gs, surv, num_turns), 'loss_history': hist, 'time': times,
+            'net': net if predictor == 'mlp' else None}
+
+
+def normal_form_4d(line, device):
+    """(inv_w4, closed orbit x, px, y, py) of the line's 4d linear normal
+    form at delta 0: the normalisation of the features."""
+    from .twiss import (_reference, compute_one_turn_matrix_finite_differences,
+                        find_closed_orbit, get_linear_normal_form)
+    if line.tracker is None:
+        line.build_tracker(device=device)
+    ref = _reference(line, None)
+    co = find_closed_orbit(line, ref, method='4d')
+    R, _ = compute_one_turn_matrix_finite_differences(line, ref, co, kill_cavity=True)
+    inv_w4 = get_linear_normal_form(R, only_4d_block=True)[1][:4, :4]
+    return inv_w4, [float(v) for v in co[:4]]
+
+
+def da_errors(da_a, da_b):
+    """Per lattice: the largest |difference| of the per-angle apertures and
+    the difference of their means (beam sigmas)."""
+    out = []
+    for a, b in zip(da_a, da_b):
+        d = np.array([a[k] - b[k] for k in b])
+        out.append({'max_abs': float(np.max(np.abs(d))),
+                    'mean_diff': float(np.mean([a[k] for k in b]) - np.mean(list(b.values())))})
+    return out
 
 
 def surrogate_da_scan(line, r_max, n_r, n_angles, num_turns, nemitt_x, nemitt_y,
                       deltas=(0.0,), feature_turns=256, train_fraction=0.15, band=1.5,
-                      epochs=150, seed=0, net=None, feature_chunk=200_000):
-    """DA scan screened by the surrogate; returns the survival turns used
-    for the aperture (exact where tracked, predicted elsewhere), the
-    per-angle apertures, which particles were tracked in full, timings and
-    the network.  The line must have a device tracker.
-
-    band: half-width, in beam sigmas, of the amplitude band around each
-    angle's predicted aperture whose particles are tracked to num_turns."""
-    torch = _t()
-    from . import da
-    from .monitor import ParticlesMonitor
-    from .twiss import (_reference, compute_one_turn_matrix_finite_differences,
-                        find_closed_orbit, get_linear_normal_form)
+                      epochs=150, seed=0, net=None):
+    """Screened scan of one line (screened_da_scan with the band in beam
+    sigmas around each angle's predicted aperture only); returns the grid,
+    the survival turns (exact where tracked, predicted elsewhere) in grid
+    order, the per-angle apertures, which particles were tracked in full,
+    timings and the network.  The line must have a device tracker."""
+    from .batch import BatchTracker
     dev = line.tracker.backend.device
-    t_all = time.perf_counter()
-    r, th, d = da.polar_grid(r_max, n_r, n_angles, deltas)
-    xn, yn = r * np.cos(th), r * np.sin(th)
-    p = da.build_normalized_particles(line, xn, 0.0, yn, 0.0, d, nemitt_x, nemitt_y)
-    p.move(_context=dev)
-    n = len(r)
-    ref = _reference(line, None)
-    bg = float(ref._np('beta0')[0] * ref._np('gamma0')[0])
-    sx, sy = math.sqrt(nemitt_x / bg), math.sqrt(nemitt_y / bg)
-    co = find_closed_orbit(line, ref, method='4d')
-    R, _ = compute_one_turn_matrix_finite_differences(line, ref, co, kill_cavity=True)
-    invW4 = get_linear_normal_form(R, only_4d_block=True)[1][:4, :4]
-    # 1. feature turns for every particle, in chunks of particle ids (the
-    # turn-by-turn record holds every field of every particle and turn)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    order = torch.argsort(p._arrays['particle_id'])
-    feat = torch.empty((n, 16), dtype=torch.float32, device=dev)
-    for c0 in range(0, n, feature_chunk):
-        c1 = min(n, c0 + feature_chunk)
-        slots = order[c0:c1]
-        sub = _take(p, slots)
-        mon = ParticlesMonitor(start_at_turn=0, stop_at_turn=feature_turns,
-                               particle_id_range=(c0, c1), auto_to_numpy=False, _context=dev)
-        line.track(sub, num_turns=feature_turns, turn_by_turn_monitor=mon)
-        feat[c0:c1] = features(mon, xn[c0:c1], yn[c0:c1], d[c0:c1], invW4, sx, sy,
-                               feature_turns)
-        for nn, vv in p._arrays.items():
-            if vv.dtype == torch.uint32:
-                vv.view(torch.int32)[slots] = sub._arrays[nn].view(torch.int32)
-            else:
-                vv[slots] = sub._arrays[nn]
-        del mon, sub
-    torch.cuda.synchronize(dev)
-    t_feat = time.perf_counter() - t0
-    surv = np.full(n, -1, dtype=np.int64)
-    st = p._arrays['state'][order].cpu().numpy()
-    at = p._arrays['at_turn'][order].cpu().numpy()
-    surv[st <= 0] = at[st <= 0]
-    alive = np.flatnonzero(st > 0)
-    exact = st <= 0
-    # 2. labels: a random subset of the survivors tracked to num_turns
-    rng = np.random.default_rng(seed)
-    n_train = max(16, int(train_fraction * len(alive))) if len(alive) else 0
-    train = np.sort(rng.choice(alive, size=min(n_train, len(alive)), replace=False))
-
-    def track_rest(ids):
-        if len(ids) == 0:
-            return np.zeros(0, dtype=np.int64)
-        sub = _take(p, order[torch.as_tensor(ids, device=dev)])
-        line.track(sub, num_turns=num_turns - feature_turns)
-        s = sub.sorted_by_id()
-        return np.where(s['state'] > 0, num_turns, s['at_turn'])
-
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    surv[train] = track_rest(train)
-    exact[train] = True
-    torch.cuda.synchronize(dev)
-    t_train_track = time.perf_counter() - t0
-    # 3. fit log(survival) on the matrix cores
-    t0 = time.perf_counter()
-    net = net or MLP(device=dev, seed=seed)
-    fx = feat[torch.as_tensor(train, device=dev)]
-    fy = torch.as_tensor(np.log(surv[train].astype(np.float64)), dtype=torch.float32,
-                         device=dev)
-    hist = net.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(train) // 4)),
-                   seed=seed)
-    pred = np.exp(np.minimum(net.predict(feat).cpu().numpy().astype(np.float64),
-                             math.log(1e12)))
-    torch.cuda.synchronize(dev)
-    t_fit = time.perf_counter() - t0
-    # 4. band around each angle's predicted aperture, tracked in full
-    guess = np.where(exact, surv, np.minimum(pred, num_turns)).astype(np.float64)
-    band_ids = []
-    for a in np.unique(th):
-        for dd in np.unique(d):
-            sel = np.flatnonzero((th == a) & (d == dd))
-            sel = sel[np.argsort(r[sel])]
-            lost = np.flatnonzero(guess[sel] < 0.5 * num_turns)
-            r_pred = r[sel][lost[0]] if len(lost) else r_max
-            inb = sel[(np.abs(r[sel] - r_pred) <= band) & ~exact[sel]]
-            band_ids.append(inb)
-    band_ids = np.sort(np.concatenate(band_ids)) if band_ids else np.zeros(0, np.int64)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    surv[band_ids] = track_rest(band_ids)
-    exact[band_ids] = True
-    torch.cuda.synchronize(dev)
-    t_band = time.perf_counter() - t0
-    # 5. the rest keeps its prediction (survives num_turns when predicted to)
-    rest = ~exact
-    surv[rest] = np.where(pred[rest] >= 0.5 * num_turns, num_turns,
-                          np.minimum(pred[rest], num_turns - 1)).astype(np.int64)
-    return {'r': r, 'theta': th, 'delta': d, 'survival': surv, 'exact': exact,
-            'tracked_full': np.union1d(train, band_ids), 'pred': pred,
-            'da': da.dynamic_aperture(r, th, d, surv, num_turns), 'loss_history': hist,
-            'time': {'features': t_feat, 'train_tracking': t_train_track, 'fit': t_fit,
-                     'band_tracking': t_band, 'total': time.perf_counter() - t_all},
-            'net': net}
+    bt = BatchTracker([line], device=dev, compile='auto')
+    gs = GridSet([line], r_max, n_r, n_angles, deltas, nemitt_x, nemitt_y, dev)
+    res = screened_da_scan(bt, gs, num_turns, feature_turns=feature_turns,
+                           label_fraction=train_fraction, band_factor=1.0, band_sigma=band,
+                           epochs=epochs, seed=seed, net=net)
+    return {'r': gs.r, 'theta': gs.theta, 'delta': gs.delta, 'survival': res['survival'],
+            'exact': res['exact'], 'tracked_full': np.union1d(res['labelled'], res['band']),
+            'da': res['da'][0], 'loss_history': res['loss_history'], 'time': res['time'],
+            'net': res['net']}
