@@ -64,3 +64,18 @@ def test_band_factor_selects_predicted_survival_near_target():
     guess[:10] = [1, 10, 200, 333, 334, 1000, 2999, 3001, 5, 500]
     band = S.boundary_band(gs, guess, rest, T, band_factor=3.0, band_sigma=-1.0)
     assert set(band[band < 10].tolist()) == {4, 5, 6, 9}
+
+
+def test_predicted_aperture_uses_half_the_turns():
+    """A group whose predictions sit just below num_turns (survivors as the
+    network sees them) has no predicted loss: the band lies at the top of
+    the grid, not at its first particle."""
+    T = 1000
+    gs = _grid(n_lines=1, deltas=(0.0,), n_angles=1)
+    guess = np.full(gs.n_grid, 0.9 * T)
+    rest = np.arange(gs.n_grid)
+    band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=0.5)
+    assert len(band) and gs.r[band].min() >= gs.r.max() - 0.5 - 1e-9
+    guess[30:] = 0.2 * T
+    band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=0.5)
+    assert np.allclose(sorted(gs.r[band]), sorted(gs.r[np.abs(gs.r - gs.r[30]) <= 0.5]))

@@ -1,6 +1,7 @@
 """The surrogate network's matrix-core products and gradients (xtm_gemm_bf16
 through xtrack_amd.surrogate) against CPU references in float64 with the
-same BF16 rounding points."""
+same BF16 rounding points: plain and transposed operands, ragged sizes,
+and long-k products that split k over blocks (weight gradients)."""
 import numpy as np
 import pytest
 
@@ -16,7 +17,11 @@ def _bf16_round(a):
 
 @pytest.mark.parametrize('M,N,K,ta,tb', [(1, 1, 1, False, False), (70, 33, 17, False, False),
                                          (130, 257, 16, True, False),
-                                         (64, 64, 300, False, True), (5, 100, 40, True, True)])
+                                         (64, 64, 300, False, True), (5, 100, 40, True, True),
+                                         (256, 256, 4096, True, False),
+                                         (16, 256, 1500, True, False),
+                                         (2000, 256, 256, False, False),
+                                         (300, 1, 256, False, False)])
 def test_gemm_against_cpu(M, N, K, ta, tb):
     import torch
     from xtrack_amd import surrogate as S
@@ -133,3 +138,41 @@ def test_surrogate_da_scan_against_plain():
     assert frac < 0.7
     assert max(errs) <= step + 1e-9
     assert res['loss_history'][-1] < res['loss_history'][0]
+
+
+def test_network_gain_over_interpolation():
+    """What the trained network gains: the same screened scan (feature
+    turns, 5 % of the survivors labelled, boundary band tracked in full)
+    with the network replaced by interpolating the labels in amplitude
+    within each angle (predictor='interp').  Both scans' tracked particles
+    keep their plain survival turns; the network's apertures are within two
+    grid steps of plain tracking and no worse than the interpolation's."""
+    import torch
+    import xtrack_amd as xt
+    from xtrack_amd import surrogate as S
+    print('[network gain]', flush=True)
+    kw = dict(r_max=12, n_r=60, n_angles=12, deltas=(0.0, 2e-3), nemitt_x=2.5e-6,
+              nemitt_y=2.5e-6)
+    line = _sext_ring()
+    line.build_tracker(device='cuda:0')
+    dev = torch.device('cuda:0')
+    bt = xt.BatchTracker([line], device=dev, compile=True)
+    gs = S.GridSet([line], device=dev, **kw)
+    T = 2000
+    plain = S.plain_da_scan_batch(bt, gs, T)
+    out = {}
+    for pred in ('mlp', 'interp'):
+        gs.reset()
+        res = S.screened_da_scan(bt, gs, T, feature_turns=128, label_fraction=0.05,
+                                 band_factor=2.0, band_sigma=0.5, epochs=100, seed=0,
+                                 predictor=pred)
+        ex = res['exact']
+        assert np.array_equal(res['survival'][ex], plain['survival'][ex])
+        err = S.da_errors(res['da'], plain['da'])[0]
+        out[pred] = (err['max_abs'], res['tracked_full_fraction'])
+        print(f'{pred}: DA error max {err["max_abs"]:.3f} sigma (mean diff '
+              f'{err["mean_diff"]:+.3f}), tracked in full {res["tracked_full_fraction"]:.3f}',
+              flush=True)
+    step = kw['r_max'] / kw['n_r']
+    assert out['mlp'][0] <= 2 * step + 1e-9
+    assert out['mlp'][0] <= out['interp'][0] + 1e-9

@@ -414,12 +414,15 @@ def interp_predictor(gs, labelled, surv, candidates):
     return pred
 
 
-def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma):
-    """The slots of `rest` to track in full: guessed survival (exact where
-    known, predicted for `rest`) within a factor band_factor of num_turns,
-    or amplitude within band_sigma of the first guessed loss of the
-    particle's (lattice, angle, delta) group (the predicted aperture; the
-    largest amplitude when no loss is guessed)."""
+def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_below=0.5):
+    """The slots of `rest` to track in full.  guess: survival per slot,
+    exact where known, predicted for the slots of `rest`.  A slot of `rest`
+    is in the band when its predicted survival lies within a factor
+    band_factor of num_turns, or its amplitude within band_sigma of the
+    predicted aperture of its (lattice, angle, delta) group: the smallest
+    amplitude guessed lost (known survival below num_turns, or predicted
+    survival below lost_below * num_turns; the largest amplitude when none
+    is)."""
     rest = np.asarray(rest, dtype=np.int64)
     pred = guess[rest]
     in_band = (pred >= num_turns / band_factor) & (pred <= num_turns * band_factor)
@@ -427,8 +430,9 @@ def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma):
     r_all = gs.r[np.arange(n_all) % gs.n_grid]
     keys = np.stack(_group_keys(gs, np.arange(n_all)), axis=1)
     gid = np.unique(keys, axis=0, return_inverse=True)[1].reshape(-1)
-    lost_r = np.full(int(gid.max()) + 1, np.inf)
     lost = guess < num_turns
+    lost[rest] = pred < lost_below * num_turns
+    lost_r = np.full(int(gid.max()) + 1, np.inf)
     np.minimum.at(lost_r, gid[lost], r_all[lost])
     r_pred = np.where(np.isfinite(lost_r), lost_r, gs.r.max())
     in_band |= np.abs(r_all[rest] - r_pred[gid[rest]]) <= band_sigma
@@ -453,7 +457,7 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
        beam sigmas of their angle's predicted aperture, are tracked to
        num_turns (the boundary band);
     5. every other particle keeps its prediction: alive if predicted to
-       survive num_turns, else lost at its predicted turn.
+       survive at least half of num_turns, else lost at its predicted turn.
 
     Tracking is continued, not restarted: a particle tracked in full has
     exactly its plain-scan survival turn.  normalisation: (inv_w4, co) of
@@ -519,10 +523,12 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     surv_t = survival_of(gs, num_turns)
     surv[band] = surv_t[band]
     exact[band] = True
-    # 5. the rest keeps its prediction
+    # 5. the rest keeps its prediction: alive when predicted to survive half
+    # the turns or more (the network regresses log survival, capped at
+    # num_turns for the survivors, so their predictions scatter around it)
     left = np.flatnonzero(~exact)
     p_left = guess[left]
-    surv[left] = np.where(p_left >= num_turns, num_turns,
+    surv[left] = np.where(p_left >= 0.5 * num_turns, num_turns,
                           np.clip(p_left, feature_turns, num_turns - 1)).astype(np.int64)
     times['total'] = time.perf_counter() - t_all
     return {'survival': surv, 'exact': exact, 'labelled': lab, 'band': band,
