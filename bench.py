@@ -513,7 +513,7 @@ def main_da(args):
     gs = surrogate.GridSet(lines, 20.0, args.da_amplitudes, args.da_angles, (0.0, 2.7e-4),
                            NEMITT, NEMITT, torch.device('cuda:0'))
     norm = surrogate.normal_form_4d(lines[0], torch.device('cuda:0'))
-    n = gs.n_grid * gs.n_lines
+    n = gs.n_slots
     log(f'da: grid of {n} particles built ({time.perf_counter() - t0:.1f} s)')
     T = args.da_turns
     plain = surrogate.plain_da_scan_batch(bt, gs, T)

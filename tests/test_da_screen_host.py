@@ -9,10 +9,16 @@ from xtrack_amd import da, surrogate as S
 
 
 def _grid(n_lines=2, r_max=10.0, n_r=40, n_angles=3, deltas=(0.0, 1e-3)):
+    """GridSet's host attributes without particles; n_r per lattice."""
     gs = types.SimpleNamespace()
-    gs.r, gs.theta, gs.delta = da.polar_grid(r_max, n_r, n_angles, deltas)
-    gs.n_grid, gs.n_lines = len(gs.r), n_lines
-    gs.starts = np.arange(n_lines + 1) * gs.n_grid
+    n_r = [n_r] * n_lines if np.ndim(n_r) == 0 else list(n_r)
+    grids = [da.polar_grid(r_max, nr, n_angles, deltas) for nr in n_r]
+    gs.r_slot = np.concatenate([g[0] for g in grids])
+    gs.theta_slot = np.concatenate([g[1] for g in grids])
+    gs.delta_slot = np.concatenate([g[2] for g in grids])
+    gs.r_max, gs.n_lines = r_max, n_lines
+    gs.starts = np.concatenate([[0], np.cumsum([len(g[0]) for g in grids])])
+    gs.n_slots = int(gs.starts[-1])
     gs.line_of = lambda slots: np.searchsorted(gs.starts, slots, side='right') - 1
     return gs
 
@@ -20,10 +26,9 @@ def _grid(n_lines=2, r_max=10.0, n_r=40, n_angles=3, deltas=(0.0, 1e-3)):
 def _truth(gs, T, da_of):
     """Survival: T below the aperture da_of(line, theta), falling as
     T * exp(-(r - da)) above it."""
-    slots = np.arange(gs.n_grid * gs.n_lines)
-    g = slots % gs.n_grid
-    ap = np.array([da_of(l, th) for l, th in zip(gs.line_of(slots), gs.theta[g])])
-    r = gs.r[g]
+    slots = np.arange(gs.n_slots)
+    ap = np.array([da_of(l, th) for l, th in zip(gs.line_of(slots), gs.theta_slot)])
+    r = gs.r_slot
     return np.where(r <= ap, T, np.floor(T * np.exp(-(r - ap)))).astype(np.int64), ap
 
 
@@ -36,7 +41,7 @@ def test_subset_layout():
 
 def test_interp_predictor_and_band_find_the_aperture():
     T = 1000
-    gs = _grid()
+    gs = _grid(n_r=(40, 55))
     da_of = lambda l, th: 4.0 + l + 2 * th
     surv, ap = _truth(gs, T, da_of)
     slots = np.arange(len(surv))
@@ -50,17 +55,17 @@ def test_interp_predictor_and_band_find_the_aperture():
     guess[rest] = pred
     band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=1.0)
     # every particle of `rest` at the true aperture's first loss is in the band
-    g = rest % gs.n_grid
-    first_loss = np.abs(gs.r[g] - ap[rest] - (gs.r[1] - gs.r[0])) < 0.75 * (gs.r[1] - gs.r[0])
-    assert np.all(np.isin(rest[first_loss & (gs.r[g] < gs.r.max())], band))
+    step = gs.r_slot[1] - gs.r_slot[0]
+    first_loss = np.abs(gs.r_slot[rest] - ap[rest] - step) < 0.75 * step
+    assert np.all(np.isin(rest[first_loss & (gs.r_slot[rest] < gs.r_max)], band))
     assert len(band) < 0.5 * len(rest)
 
 
 def test_band_factor_selects_predicted_survival_near_target():
     T = 1000
     gs = _grid(n_lines=1)
-    guess = np.full(gs.n_grid, float(T))
-    rest = np.arange(gs.n_grid)
+    guess = np.full(gs.n_slots, float(T))
+    rest = np.arange(gs.n_slots)
     guess[:10] = [1, 10, 200, 333, 334, 1000, 2999, 3001, 5, 500]
     band = S.boundary_band(gs, guess, rest, T, band_factor=3.0, band_sigma=-1.0)
     assert set(band[band < 10].tolist()) == {4, 5, 6, 9}
@@ -72,10 +77,11 @@ def test_predicted_aperture_uses_half_the_turns():
     the grid, not at its first particle."""
     T = 1000
     gs = _grid(n_lines=1, deltas=(0.0,), n_angles=1)
-    guess = np.full(gs.n_grid, 0.9 * T)
-    rest = np.arange(gs.n_grid)
+    guess = np.full(gs.n_slots, 0.9 * T)
+    rest = np.arange(gs.n_slots)
     band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=0.5)
-    assert len(band) and gs.r[band].min() >= gs.r.max() - 0.5 - 1e-9
+    assert len(band) and gs.r_slot[band].min() >= gs.r_max - 0.5 - 1e-9
     guess[30:] = 0.2 * T
     band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=0.5)
-    assert np.allclose(sorted(gs.r[band]), sorted(gs.r[np.abs(gs.r - gs.r[30]) <= 0.5]))
+    r = gs.r_slot
+    assert np.allclose(sorted(r[band]), sorted(r[np.abs(r - r[30]) <= 0.5]))

@@ -267,37 +267,48 @@ def features_from_acc(acc, x_norm0, y_norm0, delta):
 
 class GridSet:
     """The grid particles of every lattice in one device particle set:
-    lattice l holds slots starts[l]..starts[l+1], in grid order (r, theta,
-    delta as da.polar_grid flattens them), particle_id = slot."""
+    lattice l holds slots starts[l]..starts[l+1] on its own polar grid of
+    n_r[l] amplitudes (0-r_max beam sigmas) x n_angles angles x deltas
+    (da.polar_grid order), particle_id = slot.  Per slot: r, theta and
+    delta of its grid point (r_slot, theta_slot, delta_slot)."""
 
     def __init__(self, lines, r_max, n_r, n_angles, deltas, nemitt_x, nemitt_y, device):
         from . import da
         from .particles import Particles
         from .twiss import _reference
-        self.r, self.theta, self.delta = da.polar_grid(r_max, n_r, n_angles, deltas)
-        self.n_grid = len(self.r)
         self.n_lines = len(lines)
-        xn, yn = self.r * np.cos(self.theta), self.r * np.sin(self.theta)
-        hosts = []
-        for ln in lines:
-            pp = da.build_normalized_particles(ln, xn, 0.0, yn, 0.0, self.delta, nemitt_x,
-                                               nemitt_y)
+        n_r = [int(n_r)] * self.n_lines if np.ndim(n_r) == 0 else [int(v) for v in n_r]
+        if len(n_r) != self.n_lines:
+            raise ValueError('one amplitude count per lattice is needed')
+        self.r_max = float(r_max)
+        hosts, rs, ths, ds = [], [], [], []
+        for ln, nr in zip(lines, n_r):
+            r, th, d = da.polar_grid(r_max, nr, n_angles, deltas)
+            pp = da.build_normalized_particles(ln, r * np.cos(th), 0.0, r * np.sin(th), 0.0, d,
+                                               nemitt_x, nemitt_y)
             hosts.append(pp.to_numpy())
+            rs.append(r)
+            ths.append(th)
+            ds.append(d)
+        self.r_slot = np.concatenate(rs)
+        self.theta_slot = np.concatenate(ths)
+        self.delta_slot = np.concatenate(ds)
+        sizes = [len(r) for r in rs]
+        self.starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.n_slots = int(self.starts[-1])
         p0 = _reference(lines[0], None)
         big = Particles.__new__(Particles)
-        big._capacity = self.n_grid * self.n_lines
+        big._capacity = self.n_slots
         big.q0, big.mass0, big.t_sim = p0.q0, p0.mass0, 0.0
         big.start_tracking_at_element = -1
         big._arrays = {nn: np.concatenate([h[nn] for h in hosts]) for nn in hosts[0]}
-        big._arrays['particle_id'] = np.arange(big._capacity, dtype=np.int64)
+        big._arrays['particle_id'] = np.arange(self.n_slots, dtype=np.int64)
         big._device = None
         big.move(_context=device)
         self.particles = big
         self._initial = {nn: vv.clone() for nn, vv in big._arrays.items()}
-        self.starts = np.arange(self.n_lines + 1, dtype=np.int64) * self.n_grid
-        self.x_norm = np.tile(xn, self.n_lines)
-        self.y_norm = np.tile(yn, self.n_lines)
-        self.delta_all = np.tile(self.delta, self.n_lines)
+        self.x_norm = self.r_slot * np.cos(self.theta_slot)
+        self.y_norm = self.r_slot * np.sin(self.theta_slot)
         bg = float(p0._np('beta0')[0] * p0._np('gamma0')[0])
         self.sigma = (math.sqrt(nemitt_x / bg), math.sqrt(nemitt_y / bg))
         self.device = device
@@ -361,7 +372,8 @@ def da_per_line(gs, surv, num_turns):
     out = []
     for l in range(gs.n_lines):
         sl = slice(gs.starts[l], gs.starts[l + 1])
-        out.append(da.dynamic_aperture(gs.r, gs.theta, gs.delta, surv[sl], num_turns))
+        out.append(da.dynamic_aperture(gs.r_slot[sl], gs.theta_slot[sl], gs.delta_slot[sl],
+                                       surv[sl], num_turns))
     return out
 
 
@@ -372,7 +384,7 @@ def plain_da_scan_batch(bt, gs, num_turns, block=256):
     torch = _t()
     torch.cuda.synchronize(gs.device)
     t0 = time.perf_counter()
-    track_subset(bt, gs, np.arange(gs.n_grid * gs.n_lines), num_turns, block=block)
+    track_subset(bt, gs, np.arange(gs.n_slots), num_turns, block=block)
     wall = time.perf_counter() - t0
     surv = survival_of(gs, num_turns)
     return {'survival': surv, 'da': da_per_line(gs, surv, num_turns), 'time': wall}
@@ -380,9 +392,8 @@ def plain_da_scan_batch(bt, gs, num_turns, block=256):
 
 def _group_keys(gs, slots):
     """(lattice, angle index, delta index) of each slot."""
-    g = slots % gs.n_grid
-    th_idx = np.searchsorted(np.unique(gs.theta), gs.theta[g])
-    d_idx = np.searchsorted(np.unique(gs.delta), gs.delta[g])
+    th_idx = np.searchsorted(np.unique(gs.theta_slot), gs.theta_slot[slots])
+    d_idx = np.searchsorted(np.unique(gs.delta_slot), gs.delta_slot[slots])
     return gs.line_of(slots), th_idx, d_idx
 
 
@@ -407,9 +418,9 @@ def interp_predictor(gs, labelled, surv, candidates):
         if not g:
             continue
         g = np.asarray(g)
-        g = g[np.argsort(gs.r[g % gs.n_grid])]
+        g = g[np.argsort(gs.r_slot[g])]
         m = np.asarray(members)
-        pred[m] = np.exp(np.interp(gs.r[cand[m] % gs.n_grid], gs.r[g % gs.n_grid],
+        pred[m] = np.exp(np.interp(gs.r_slot[cand[m]], gs.r_slot[g],
                                    np.log(np.maximum(surv[g], 1))))
     return pred
 
@@ -427,14 +438,14 @@ def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_belo
     pred = guess[rest]
     in_band = (pred >= num_turns / band_factor) & (pred <= num_turns * band_factor)
     n_all = len(guess)
-    r_all = gs.r[np.arange(n_all) % gs.n_grid]
+    r_all = gs.r_slot
     keys = np.stack(_group_keys(gs, np.arange(n_all)), axis=1)
     gid = np.unique(keys, axis=0, return_inverse=True)[1].reshape(-1)
     lost = guess < num_turns
     lost[rest] = pred < lost_below * num_turns
     lost_r = np.full(int(gid.max()) + 1, np.inf)
     np.minimum.at(lost_r, gid[lost], r_all[lost])
-    r_pred = np.where(np.isfinite(lost_r), lost_r, gs.r.max())
+    r_pred = np.where(np.isfinite(lost_r), lost_r, gs.r_max)
     in_band |= np.abs(r_all[rest] - r_pred[gid[rest]]) <= band_sigma
     return rest[in_band]
 
@@ -465,7 +476,7 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     torch = _t()
     dev = gs.device
     t_all = time.perf_counter()
-    n_all = gs.n_grid * gs.n_lines
+    n_all = gs.n_slots
     if normalisation is None:
         normalisation = normal_form_4d(bt.lines[0], dev)
     times = {}
@@ -473,7 +484,7 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     acc = track_subset(bt, gs, np.arange(n_all), feature_turns, features=normalisation)
-    feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_all)
+    feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_slot)
     torch.cuda.synchronize(dev)
     times['features'] = time.perf_counter() - t0
     surv = survival_of(gs, feature_turns)
@@ -577,7 +588,8 @@ def surrogate_da_scan(line, r_max, n_r, n_angles, num_turns, nemitt_x, nemitt_y,
     res = screened_da_scan(bt, gs, num_turns, feature_turns=feature_turns,
                            label_fraction=train_fraction, band_factor=1.0, band_sigma=band,
                            epochs=epochs, seed=seed, net=net)
-    return {'r': gs.r, 'theta': gs.theta, 'delta': gs.delta, 'survival': res['survival'],
+    return {'r': gs.r_slot, 'theta': gs.theta_slot, 'delta': gs.delta_slot,
+            'survival': res['survival'],
             'exact': res['exact'], 'tracked_full': np.union1d(res['labelled'], res['band']),
             'da': res['da'][0], 'loss_history': res['loss_history'], 'time': res['time'],
             'net': res['net']}
