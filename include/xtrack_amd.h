@@ -457,6 +457,21 @@ int32_t xtm_gemm_bf16(const uint16_t *A, int64_t sam, int64_t sak, const uint16_
                       const float *bias, int32_t relu, int64_t M, int64_t N, int64_t K,
                       void *stream);
 
+/* xtm_gemm_bf16 without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same kernels
+ * and results as xtm_gemm_bf16; the k-split partials go to a file-scope
+ * device array of xtm_gemm_bf16_scratch_floats() floats (2^20, 4 MiB) per
+ * GPU instead of a library-allocated workspace, and products whose
+ * partials would not fit are rejected (XTM_ERR_ARG, message in
+ * xtm_last_error).  Calls that overlap on one GPU share the array and
+ * race. */
+int32_t xtm_gemm_bf16_noalloc(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B,
+                              int64_t sbk, int64_t sbn, float *C, int64_t ldc, uint16_t *Cb,
+                              const float *bias, int32_t relu, int64_t M, int64_t N, int64_t K,
+                              void *stream);
+int64_t xtm_gemm_bf16_scratch_floats(void);
+
 /* Wait for all work queued on `stream`. */
 int32_t xtm_synchronize(void *stream);
 

@@ -203,6 +203,9 @@ def load():
     lib.xtm_gemm_bf16.argtypes = [c.c_void_p, c.c_int64, c.c_int64, c.c_void_p, c.c_int64,
                                   c.c_int64, c.c_void_p, c.c_int64, c.c_void_p, c.c_void_p,
                                   c.c_int32, c.c_int64, c.c_int64, c.c_int64, c.c_void_p]
+    lib.xtm_gemm_bf16_noalloc.restype = c.c_int32
+    lib.xtm_gemm_bf16_noalloc.argtypes = lib.xtm_gemm_bf16.argtypes
+    lib.xtm_gemm_bf16_scratch_floats.restype = c.c_int64
     lib.xtm_synchronize.restype = c.c_int32
     lib.xtm_synchronize.argtypes = [c.c_void_p]
     lib.xtm_math_eval.restype = c.c_int32

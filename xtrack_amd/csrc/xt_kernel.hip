@@ -163,6 +163,10 @@ int32_t hip_fail(hipError_t e, const char *where) {
 
 }  // namespace
 
+/* the message xtm_last_error() returns, for the library's other source
+   files (matrix products) */
+int32_t xt_report_error(int32_t code, const char *msg) { return fail(code, msg); }
+
 struct xtm_line {
     int32_t device;
     int64_t num_elements;

@@ -22,14 +22,17 @@
 // over a batch) split k over up to 256 / tiles blocks (grid z); each block
 // writes its partial tile to a workspace and a second kernel sums the
 // partials in split order (deterministic) and applies bias, ReLU and the
-// BF16 rounding.  The workspace is at most GEMM_WS_FLOATS floats; the
-// library allocates it once per device.
+// BF16 rounding.  The workspace is at most GEMM_WS_FLOATS floats: the
+// library allocates it once per device for xtm_gemm_bf16;
+// xtm_gemm_bf16_noalloc uses a file-scope device array instead.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <mutex>
 
 #include "../../include/xtrack_amd.h"
+
+int32_t xt_report_error(int32_t code, const char *msg);   // xt_kernel.hip
 
 namespace {
 
@@ -169,11 +172,14 @@ int32_t gemm_launch(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t 
                     int64_t kc) {
     const int64_t gy = (M + TM - 1) / TM, gx = (N + TN - 1) / TN;
     if (splits > 1) {
-        if (!ws || splits * M * N > ws_floats) return XTM_ERR_ARG;
+        if (!ws || splits * M * N > ws_floats)
+            return xt_report_error(XTM_ERR_ARG, "xtm_gemm_bf16: split partials exceed the "
+                                                "workspace");
         hipLaunchKernelGGL(gemm_bf16_kernel, dim3((unsigned)gx, (unsigned)gy, (unsigned)splits),
                            dim3(256), 0, stream, A, sam, sak, B, sbk, sbn, C, ldc, Cb, bias,
                            relu, M, N, K, kc, ws);
-        if (hipGetLastError() != hipSuccess) return XTM_ERR_HIP;
+        if (hipGetLastError() != hipSuccess)
+            return xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16: kernel launch failed");
         const int64_t nb = (M * N + 255) / 256;
         hipLaunchKernelGGL(gemm_reduce_kernel, dim3((unsigned)nb), dim3(256), 0, stream, ws,
                            splits, C, ldc, Cb, bias, relu, M, N);
@@ -182,14 +188,19 @@ int32_t gemm_launch(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t 
                            stream, A, sam, sak, B, sbk, sbn, C, ldc, Cb, bias, relu, M, N, K,
                            K > 0 ? K : 1, (float *)nullptr);
     }
-    return hipGetLastError() == hipSuccess ? XTM_OK : XTM_ERR_HIP;
+    return hipGetLastError() == hipSuccess
+               ? XTM_OK
+               : xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16: kernel launch failed");
 }
 
 int32_t gemm_check(const uint16_t *A, const uint16_t *B, const float *C, int64_t ldc,
                    int64_t M, int64_t N, int64_t K) {
-    if (!A || !B || !C || M < 0 || N < 0 || K < 0 || ldc < N) return XTM_ERR_ARG;
+    if (!A || !B || !C || M < 0 || N < 0 || K < 0 || ldc < N)
+        return xt_report_error(XTM_ERR_ARG, "xtm_gemm_bf16: null operand, negative size or "
+                                            "ldc < N");
     const int64_t gy = (M + TM - 1) / TM, gx = (N + TN - 1) / TN;
-    if (gy > 65535 || gx > 0x7fffffff) return XTM_ERR_ARG;
+    if (gy > 65535 || gx > 0x7fffffff)
+        return xt_report_error(XTM_ERR_ARG, "xtm_gemm_bf16: M above 65535 * 64 rows");
     return XTM_OK;
 }
 
@@ -197,6 +208,17 @@ int32_t gemm_check(const uint16_t *A, const uint16_t *B, const float *C, int64_t
 constexpr int kMaxDevices = 64;
 std::mutex g_ws_mutex;
 float *g_ws[kMaxDevices] = {};
+
+/* Scratch of xtm_gemm_bf16_noalloc: one copy per GPU, held until the
+   process ends.  In host code `scratch` names a host-side stand-in, not a
+   device address: its device address is taken with hipGetSymbolAddress
+   once per device, with that device current, and only that pointer
+   (g_scratch[device]) is handed to the kernels.  Every call on a GPU
+   shares the array; split partials are written before they are read, so
+   nothing needs clearing between calls, and calls that overlap on one GPU
+   race. */
+__device__ __align__(256) float scratch[GEMM_WS_FLOATS];
+float *g_scratch[kMaxDevices] = {};
 
 }  // namespace
 
@@ -214,16 +236,49 @@ int32_t xtm_gemm_bf16(const uint16_t *A, int64_t sam, int64_t sak, const uint16_
     if (splits > 1) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
-            return XTM_ERR_HIP;
+            return xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16: no current device");
         std::lock_guard<std::mutex> lock(g_ws_mutex);
         if (!g_ws[dev] && hipMalloc(&g_ws[dev], GEMM_WS_FLOATS * sizeof(float)) != hipSuccess) {
             g_ws[dev] = nullptr;
-            return XTM_ERR_HIP;
+            return xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16: workspace allocation failed");
         }
         ws = g_ws[dev];
     }
     return gemm_launch(A, sam, sak, B, sbk, sbn, C, ldc, Cb, bias, relu, M, N, K,
                        (hipStream_t)stream, ws, GEMM_WS_FLOATS, splits, kc);
 }
+
+int32_t xtm_gemm_bf16_noalloc(const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B,
+                              int64_t sbk, int64_t sbn, float *C, int64_t ldc, uint16_t *Cb,
+                              const float *bias, int32_t relu, int64_t M, int64_t N, int64_t K,
+                              void *stream) {
+    int32_t st = gemm_check(A, B, C, ldc, M, N, K);
+    if (st != XTM_OK || M == 0 || N == 0) return st;
+    int64_t kc;
+    const int64_t splits = gemm_splits(M, N, K, &kc);
+    float *ws = nullptr;
+    if (splits > 1) {
+        if (splits * M * N > GEMM_WS_FLOATS)
+            return xt_report_error(XTM_ERR_ARG, "xtm_gemm_bf16_noalloc: split partials exceed "
+                                                "the scratch array (2^20 floats)");
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+            return xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16_noalloc: no current device");
+        std::lock_guard<std::mutex> lock(g_ws_mutex);
+        if (!g_scratch[dev]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return xt_report_error(XTM_ERR_HIP, "xtm_gemm_bf16_noalloc: "
+                                                    "hipGetSymbolAddress(scratch) failed");
+            g_scratch[dev] = static_cast<float *>(p);
+        }
+        ws = g_scratch[dev];
+    }
+    return gemm_launch(A, sam, sak, B, sbk, sbn, C, ldc, Cb, bias, relu, M, N, K,
+                       (hipStream_t)stream, ws, GEMM_WS_FLOATS, splits, kc);
+}
+
+int64_t xtm_gemm_bf16_scratch_floats(void) { return GEMM_WS_FLOATS; }
 
 }  // extern "C"
