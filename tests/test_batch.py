@@ -38,22 +38,23 @@ def test_block_table():
     try:
         counts = np.array([300, 0, 5, 1024, 257])
         starts = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        _native.check(lib.xtm_batch_set_layout(b, starts.ctypes.data, 256, 4))
-        n = lib.xtm_batch_num_blocks(b)
-        tab = np.zeros((n, 3), dtype=np.int64)
-        _native.check(lib.xtm_batch_copy_blocks(b, tab.ctypes.data))
-        line, first, cnt = tab.T
-        assert n == sum(-(-c // 256) for c in counts)
-        assert np.all((cnt >= 1) & (cnt <= 256))
-        covered = np.zeros(starts[-1], dtype=int)
-        for ll, f, c in tab:
-            assert starts[ll] <= f and f + c <= starts[ll + 1]
-            covered[f:f + c] += 1
-        assert np.all(covered == 1)
         nele = np.array([len(ln.elements) for ln in lines])
-        assert np.all(np.diff(nele[line]) <= 0)          # longer lines first
+        for bs in (64, 128, 256):
+            _native.check(lib.xtm_batch_set_layout(b, starts.ctypes.data, bs, 4))
+            n = lib.xtm_batch_num_blocks(b)
+            tab = np.zeros((n, 3), dtype=np.int64)
+            _native.check(lib.xtm_batch_copy_blocks(b, tab.ctypes.data))
+            line, first, cnt = tab.T
+            assert n == sum(-(-c // bs) for c in counts)
+            assert np.all((cnt >= 1) & (cnt <= bs))
+            covered = np.zeros(starts[-1], dtype=int)
+            for ll, f, c in tab:
+                assert starts[ll] <= f and f + c <= starts[ll + 1]
+                covered[f:f + c] += 1
+            assert np.all(covered == 1)
+            assert np.all(np.diff(nele[line]) <= 0)          # longer lines first
         # bad layouts and kernel choices are rejected
-        for bs, w in ((128, 4), (256, 2), (256, 6)):
+        for bs, w in ((96, 4), (32, 4), (512, 4), (256, 2), (256, 6)):
             assert lib.xtm_batch_set_layout(b, starts.ctypes.data, bs, w) == _native.XTM_ERR_ARG
         bad = starts.copy()
         bad[2] = bad[1] - 1

@@ -90,6 +90,26 @@ def test_batch_bit_exact_uneven_variants(compiled):
     print(f'lost={lost}', flush=True)
 
 
+@pytest.mark.parametrize('block', [64, 128])
+def test_batch_block_sizes_bit_exact(block):
+    """The compiled batch code and the interpreter after it at 64 and 128
+    threads per block (the per-thread LDS slots sized by the launch):
+    every variant as the oracle leaves it."""
+    print(f'[batch] block size {block}', flush=True)
+    lines = _variants()
+    parts = _same_species(_particles())
+    ref = _oracle(_variants(), [p.copy() for p in parts], 4)
+    bt = xt.BatchTracker(lines, device='cuda:0', compile=True)
+    bt.config = (block, 5)
+    got = [p.copy() for p in parts]
+    bt.track(got, num_turns=4, turns_per_launch=2)
+    assert int(bt.blocks()[:, 2].max()) <= block
+    for k, (g, r) in enumerate(zip(got, ref)):
+        if COUNTS[k]:
+            bad = compare_exact(g, r)
+            assert not bad, (k, bad)
+
+
 def test_batch_graph_replays_match_eager():
     """One launch of 2 turns captured in a HIP graph and replayed three
     times, with eager work and synchronisation between the replays; after
@@ -138,6 +158,7 @@ def test_batch_tuner_kernel_instances_agree():
     bt = xt.BatchTracker(lines, device='cuda:0')
     cfg = bt.tune(parts, turns=1)
     assert cfg in xt.batch.CANDIDATES and set(bt.tune_timings) == set(xt.batch.CANDIDATES)
+    assert {c[0] for c in xt.batch.CANDIDATES} == {64, 128, 256}
     results = {}
     for cand in xt.batch.CANDIDATES:
         bt.config = cand

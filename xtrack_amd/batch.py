@@ -4,10 +4,10 @@ The reference tracks one line per ``Line.track`` call, one ``track_line``
 launch per call (xtrack/tracker.py:1372); a study over lattice variants
 (error seeds, strength scans) loops over lines.  ``BatchTracker`` puts the
 particles of all variants into one particle set, each variant's particles
-in a contiguous range, and tracks them in one launch: every 256-thread
-block holds particles of one variant only,
-so element dispatch stays uniform within a wave, and blocks of longer
-lines are dispatched first.  Each variant's particles end exactly as
+in a contiguous range, and tracks them in one launch: every block (64, 128
+or 256 threads) holds particles of one variant only, so element dispatch
+stays uniform within a wave, and blocks of longer lines are dispatched
+first.  Each variant's particles end exactly as
 ``line.track(particles, num_turns=n)`` leaves them (full turns from
 element 0, s reset at the end of each turn, no monitor).
 
@@ -17,16 +17,18 @@ code.  With ``compile=True`` the batch is compiled once for all its lines
 with one function per distinct element *structure* (op sequence, with the
 constants read from each line's program), so that every variant runs the
 same compiled code; particles with chi != 1 follow on the interpreter.
-On the 64-variant ring study this is no faster than the interpreter (the
-constants come from memory through the scalar cache either way, and the
-variants' programs compete for it), so the interpreter is the default.
 
 Turns run in launches of ``turns_per_launch``; with ``use_graph`` one such
 launch is captured in a HIP graph (torch.cuda.CUDAGraph) and replayed.  The
-autotuner (``tune``) times one turn of each interpreter kernel instance
-(occupancy 3, 4 or 5 waves per SIMD at 256 threads per block) on a copy of
-the particles and keeps the fastest; the compiled batch code is built for 5
-waves per SIMD.
+autotuner (``tune``) times one turn of each (block size, waves per SIMD)
+candidate on a copy of the particles and keeps the fastest: block sizes 64,
+128 and 256 (smaller blocks leave fewer idle lanes in the last block of
+each variant and spread a small particle set over more compute units),
+occupancy 3, 4 or 5 waves per SIMD for the interpreter kernel.  The
+compiled batch code is built for 5 waves per SIMD, so for a compiled batch
+only the block size is searched.  The per-thread LDS slots of the particle
+state are sized by the launch (block size x 48 bytes), so every block size
+runs the same code.
 """
 import ctypes
 
@@ -38,7 +40,10 @@ from .particles import Particles
 from .tracker import TrackFlags, _LineHandle
 
 BLOCK = 256
-CANDIDATES = ((BLOCK, 3), (BLOCK, 4), (BLOCK, 5))
+BLOCK_SIZES = (64, 128, 256)
+# interpreter kernel: block size x occupancy; compiled code: block size
+CANDIDATES = tuple((bs, w) for bs in BLOCK_SIZES for w in (3, 4, 5))
+COMPILED_CANDIDATES = tuple((bs, 5) for bs in BLOCK_SIZES)
 
 
 class BatchTracker:
@@ -143,10 +148,15 @@ class BatchTracker:
         return out
 
     # ------------------------------------------------------------------
-    def tune(self, particles_list, turns=1, candidates=CANDIDATES):
+    def candidates(self):
+        """The (block size, waves per SIMD) configurations the tuner tries."""
+        return COMPILED_CANDIDATES if self.is_compiled else CANDIDATES
+
+    def tune(self, particles_list, turns=1, candidates=None):
         """Time `turns` turns of each candidate (block size, waves per SIMD)
         on a copy of the particles; keep the fastest as self.config."""
         import torch
+        candidates = self.candidates() if candidates is None else candidates
         big, starts = self._concat(particles_list)
         timings = {}
         for cand in candidates:

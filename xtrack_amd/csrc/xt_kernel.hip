@@ -34,6 +34,10 @@
 namespace {
 
 constexpr int BLOCK = 256;
+/* dynamic LDS of a tracking launch of n threads per block: the per-thread
+   slots of the particle state (xt_ops.h, track_particle_loop) */
+constexpr unsigned lds_bytes(int n) { return (unsigned)n * xt::XT_LDS_BYTES_PER_THREAD; }
+constexpr unsigned kLdsLine = lds_bytes(BLOCK);
 
 /* Occupancy target (waves per SIMD).  Four waves cap the tracking kernel
    at 128 VGPRs; on MI355X this beat both the unconstrained allocation
@@ -85,13 +89,12 @@ struct BatchBlock {
    (element programs entries[line], progs[line]) for num_turns full turns.
    The block's line is uniform across the block, so element dispatch stays
    a scalar branch as in track_kernel. */
-template <int BS, int WPE>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void batch_kernel(
+template <int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void batch_kernel(
     const BatchBlock *__restrict__ blocks, const int64_t *const *__restrict__ entries,
     const double *const *__restrict__ progs, const int64_t *__restrict__ nele,
     const double *__restrict__ lengths, const xtm_particles *__restrict__ Pd,
     int32_t num_turns, uint64_t flags, int32_t skip_chi1) {
-    static_assert(BS <= BLOCK, "the per-thread LDS slots are laid out for 256 threads");
     const BatchBlock b = xt::cptr(blocks)[blockIdx.x];
     if ((int32_t)threadIdx.x >= b.count) return;
     const xtm_particles &P = *Pd;
@@ -515,11 +518,11 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
         uint64_t a_flags = track_flags;
         void *args[] = {&rec, &entry, &prog, &line_length, &Pv, &a_turns, &a_start, &a_n,
                         &a_end, &a_reset, &a_mon, &Md, &Pd, &a_flags};
-        e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, 0,
+        e = hipModuleLaunchKernel(L->kernel, (unsigned)nblocks, 1, 1, BLOCK, 1, 1, kLdsLine,
                                   (hipStream_t)stream, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_line_kernel launch");
         /* particles with chi != 1 on the interpreter, same stream */
-        hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
+        hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), kLdsLine,
                            (hipStream_t)stream, L->d_entry, L->d_prog, L->line_length,
                            (int64_t)L->num_elements, Pd, num_turns, ele_start, num_ele_track,
                            flag_end_turn_actions, flag_reset_s_at_end_turn, flag_monitor, Md,
@@ -530,7 +533,7 @@ static int32_t track_line_impl(xtm_line *L, const xtm_particles *P, int32_t num_
     }
     /* backtracking always runs on the interpreter, with the backtracking
        programs */
-    hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), 0,
+    hipLaunchKernelGGL(track_kernel, dim3((unsigned)nblocks), dim3(BLOCK), kLdsLine,
                        (hipStream_t)stream, backtrack ? L->d_entry_bt : L->d_entry, L->d_prog,
                        L->line_length, (int64_t)L->num_elements, Pd,
                        num_turns, ele_start, num_ele_track, flag_end_turn_actions,
@@ -670,9 +673,10 @@ int32_t xtm_batch_create(xtm_line *const *lines, int32_t num_lines, xtm_batch **
 int32_t xtm_batch_set_layout(xtm_batch *B, const int64_t *line_start, int32_t block_size,
                              int32_t waves_per_eu) {
     if (!B || !line_start) return fail(XTM_ERR_ARG, "xtm_batch_set_layout: null argument");
-    if (block_size != 256 || waves_per_eu < 3 || waves_per_eu > 5)
+    if ((block_size != 64 && block_size != 128 && block_size != 256) || waves_per_eu < 3 ||
+        waves_per_eu > 5)
         return fail(XTM_ERR_ARG, "xtm_batch_set_layout: no kernel for this block size and "
-                                 "occupancy (256 threads, 3 to 5 waves per SIMD)");
+                                 "occupancy (64, 128 or 256 threads, 3 to 5 waves per SIMD)");
     const size_t nl = B->lines.size();
     if (line_start[0] != 0) return fail(XTM_ERR_ARG, "xtm_batch_set_layout: line_start[0] != 0");
     std::vector<BatchBlock> blocks;
@@ -762,19 +766,22 @@ static int32_t batch_track_impl(xtm_batch *B, const xtm_particles *P, int32_t nu
         uint64_t a_flags = track_flags;
         void *args[] = {&blocks, &recs, &entries, &progs, &nele, &lengths, &Pd, &a_turns,
                         &a_flags};
-        e = hipModuleLaunchKernel(B->kernel, grid.x, 1, 1, BLOCK, 1, 1, 0, s, args, nullptr);
+        e = hipModuleLaunchKernel(B->kernel, grid.x, 1, 1, (unsigned)B->block_size, 1, 1,
+                                  lds_bytes(B->block_size), s, args, nullptr);
         if (e != hipSuccess) return hip_fail(e, "xtm_batch_line_kernel launch");
         skip = 1;   // the interpreter follows for the particles with chi != 1
     }
+    const dim3 block((unsigned)B->block_size);
+    const unsigned lds = lds_bytes(B->block_size);
     if (B->waves_per_eu == 5)
-        hipLaunchKernelGGL((batch_kernel<256, 5>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
+        hipLaunchKernelGGL((batch_kernel<5>), grid, block, lds, s, B->d_blocks, entries, progs,
+                           B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else if (B->waves_per_eu == 4)
-        hipLaunchKernelGGL((batch_kernel<256, 4>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
+        hipLaunchKernelGGL((batch_kernel<4>), grid, block, lds, s, B->d_blocks, entries, progs,
+                           B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     else
-        hipLaunchKernelGGL((batch_kernel<256, 3>), grid, dim3(256), 0, s, B->d_blocks, entries,
-                           progs, B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
+        hipLaunchKernelGGL((batch_kernel<3>), grid, block, lds, s, B->d_blocks, entries, progs,
+                           B->d_nele, B->d_lengths, B->d_p, num_turns, track_flags, skip);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "batch_kernel launch");
     return XTM_OK;

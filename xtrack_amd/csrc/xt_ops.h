@@ -67,6 +67,8 @@ enum { KF_P1 = 1, KF_P2 = 2, KF_P3 = 4, KF_HL = 8, KF_G0 = 16, KF_H1 = 32 };
    element index (track_particle). */
 /* slots of the energy fields in PState::slow */
 enum { XS_DELTA = 0, XS_RPP = 1, XS_RVV = 2, XS_INV_RVV = 3, XS_S = 4, XS_ZETA0 = 5, XS_N = 6 };
+/* dynamic LDS bytes per thread that a tracking kernel launch must provide */
+constexpr int XT_LDS_BYTES_PER_THREAD = XS_N * 8;
 
 template <class R>
 struct PState {
@@ -1382,10 +1384,14 @@ XT_HD void track_particle_loop(const xtm_particles &P, int64_t i, const Runner &
                                int32_t flag_reset_s, int32_t flag_monitor, const MonRef &mon) {
     PState<double> p;
 #if defined(__HIP_DEVICE_COMPILE__)
-    /* the kernels run 256 threads per block */
-    __shared__ double slow_lds[XS_N * 256];
-    p.slow = slow_lds + threadIdx.x;
-    p.slow_stride = 256;
+    /* dynamic LDS of the launch, XT_LDS_BYTES_PER_THREAD bytes per thread
+       of the block (a multiple of 64 threads): wave-major, each wave's
+       XS_N slots of 64 consecutive doubles, so a slot access is one
+       conflict-free 64-lane read at a constant offset, whatever the block
+       size */
+    extern __shared__ double slow_lds[];
+    p.slow = slow_lds + (threadIdx.x >> 6) * (XS_N * 64) + (threadIdx.x & 63);
+    p.slow_stride = 64;
 #else
     double slow_buf[XS_N];
     p.slow = slow_buf;

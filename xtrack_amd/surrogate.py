@@ -138,29 +138,36 @@ class MLP:
             vh = st['v'][k] / (1 - beta2 ** st['t'])
             p.sub_(lr * mh / (vh.sqrt() + eps))
 
-    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0):
+    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0, censored=None):
         """Mean-squared error on standardised inputs and targets; x (N, 16),
-        y (N,) torch FP32 on the device.  Returns the loss per epoch."""
+        y (N,) torch FP32 on the device.  censored: None, or a bool (N,)
+        tensor marking targets that are lower bounds (particles alive at the
+        last tracked turn): their error counts only where the output lies
+        below the target, so the network may place them beyond it.  Returns
+        the loss per epoch."""
         torch = _t()
         self.x_mean = x.mean(dim=0)
         self.x_std = x.std(dim=0).clamp_min(1e-12)
         self.y_mean, self.y_std = float(y.mean()), float(y.std().clamp_min(1e-12))
         xn = (x - self.x_mean) / self.x_std
         yn = (y - self.y_mean) / self.y_std
+        cens = None if censored is None else censored.to(self.device, torch.bool)
         g = torch.Generator(device='cpu').manual_seed(seed)
         hist = []
         n = len(xn)
         for _ in range(epochs):
             perm = torch.randperm(n, generator=g).to(self.device)
-            tot = 0.0
+            tot = torch.zeros((), dtype=torch.float32, device=self.device)
             for s in range(0, n, batch):
                 idx = perm[s:s + batch]
                 out = self.forward(xn[idx], keep=True)
                 err = out - yn[idx]
+                if cens is not None:
+                    err = torch.where(cens[idx] & (err > 0), torch.zeros_like(err), err)
                 gW, gb = self.backward(2.0 * err / len(idx))
                 self.adam_step(gW, gb, lr=lr)
-                tot += float((err * err).sum())
-            hist.append(tot / n)
+                tot += (err * err).sum()
+            hist.append(float(tot) / n)
         return hist
 
     def predict(self, x, batch=65536):
@@ -461,7 +468,8 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     2. a random label_fraction of the survivors of each lattice is tracked
        to num_turns: their survival turns are the labels;
     3. the network (16-256-256-256-1, BF16 products on the matrix cores) is
-       fit to log survival turn on the labels (predictor='interp': no
+       fit to log survival turn on the labels, the survivors' turn count
+       taken as a lower bound (censored loss) (predictor='interp': no
        network, per-angle interpolation of the labels in amplitude);
     4. the survivors whose predicted survival lies within a factor
        band_factor of num_turns, or whose amplitude lies within band_sigma
@@ -513,8 +521,9 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
         fx = feat[torch.as_tensor(lab, device=dev)]
         fy = torch.as_tensor(np.log(surv[lab].astype(np.float64)), dtype=torch.float32,
                              device=dev)
+        cens = torch.as_tensor(surv[lab] >= num_turns, device=dev)
         hist = net.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(lab) // 4)),
-                       seed=seed)
+                       seed=seed, censored=cens)
         pred = np.exp(np.minimum(net.predict(feat[torch.as_tensor(rest, device=dev)])
                                  .cpu().numpy().astype(np.float64), math.log(1e12)))
     elif predictor == 'interp':
