@@ -86,13 +86,22 @@ def parse():
     ap.add_argument('--waves-per-eu', type=int, default=0,
                     help='occupancy target of the compiled kernel (0: library default)')
     ap.add_argument('--rematch', action='store_true')
+    ap.add_argument('--ring-error-seed', type=int, default=None,
+                    help='track the ring with the b3..b6 error set of this seed on every '
+                         'magnet (lattices.collider_with_errors): structure code with '
+                         'constants loaded from the element programs, for profiling')
+    ap.add_argument('--ring-error-scale', type=float, default=1e-4)
     ap.add_argument('--da-angles', type=int, default=45)
     ap.add_argument('--da-amplitudes', type=int, default=400)
     ap.add_argument('--da-turns', type=int, default=2000)
     ap.add_argument('--da-seeds', type=int, default=5)
-    ap.add_argument('--da-feature-turns', type=int, default=128)
-    ap.add_argument('--da-label-fraction', type=float, default=0.02)
-    ap.add_argument('--da-band-sigma', type=float, default=0.5)
+    ap.add_argument('--da-feature-turns', type=int, default=64)
+    ap.add_argument('--da-label-fraction', type=float, default=0.03)
+    ap.add_argument('--da-band-sigma', type=float, default=0.3)
+    ap.add_argument('--da-grid-sizes', default=None,
+                    help='lo,hi: particles per seed drawn log-uniformly in [lo, hi] (seeded), '
+                         'amplitude count per seed = particles / (angles x 2); default: '
+                         '--da-amplitudes for every seed')
     ap.add_argument('--da-epochs', type=int, default=100)
     ap.add_argument('--da-error-scale', type=float, default=3e-3,
                     help='relative strength of the b3..b6 errors (at 1e-4 the aperture of '
@@ -100,7 +109,7 @@ def parse():
     ap.add_argument('--da-predictors', default='mlp',
                     help='comma-separated: mlp (the network), interp (the same screening '
                          'with per-angle interpolation of the labels instead)')
-    ap.add_argument('--workload', choices=('collider', 'batch', 'da'), default='collider',
+    ap.add_argument('--workload', choices=('collider', 'batch', 'da', 'mlp'), default='collider',
                     help='collider: the thick ring (the headline line); batch: 64 lattice '
                          'variants of it with uneven particle counts in one launch; da: '
                          'dynamic-aperture scan of error seeds of the ring, plain against '
@@ -478,6 +487,75 @@ def batch_leg(args, steps, kernels=None):
     return out
 
 
+def main_mlp(args):
+    """The survival network's products on the matrix cores (xtm_gemm_bf16,
+    BF16 operands, FP32 accumulation, fused bias, ReLU and BF16 rounding of
+    the output): the hidden-layer product of inference over a DA grid,
+    (65,536 x 256) x (256 x 256), timed with HIP events on the launch
+    stream over `--steps` launches, and one training step of the
+    16-256-256-256-1 network at the minibatch of the fit (1,024 samples:
+    forward, backward and Adam).  The product moves 32 MiB of BF16 input,
+    64 MiB of FP32 and 32 MiB of BF16 output per launch for 8.6 GFLOP
+    (64 flop per byte, below the 312 flop per byte where MI355X's BF16
+    matrix peak and its HBM bandwidth meet), so it is bound by HBM."""
+    import torch
+    from xtrack_amd import surrogate as S
+    dev = torch.device('cuda:0')
+    g = torch.Generator(device='cpu').manual_seed(0)
+    M, K, N = 65536, 256, 256
+    a = S.to_bf16(torch.randn(M, K, generator=g).to(dev))
+    w = S.to_bf16((torch.randn(K, N, generator=g) / 16).to(dev))
+    bias = torch.zeros(N, dtype=torch.float32, device=dev)
+    for _ in range(max(args.warmup, 1)):
+        S.gemm(a, w, bias=bias, relu=True, want_bf16=True)
+    stream = torch.cuda.current_stream()
+    steps = max(args.steps, 1)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        S.gemm(a, w, bias=bias, relu=True, want_bf16=True)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [x.elapsed_time(y) for x, y in ev]
+    ms_mean = float(np.mean(ms))
+    flop = 2.0 * M * N * K
+    bytes_ = 2.0 * M * K + 2.0 * K * N + 4.0 * N + (4.0 + 2.0) * M * N
+    # one training step at the fit's minibatch
+    net = S.MLP(device=dev, seed=0)
+    xb = torch.randn(1024, 16, generator=g).to(dev)
+    yb = torch.randn(1024, generator=g).to(dev)
+    net.fit(xb, yb, epochs=2, batch=1024, seed=0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    net.fit(xb, yb, epochs=10, batch=1024, seed=0)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    train_ms = e0.elapsed_time(e1) / 10
+    achieved = bytes_ / (ms_mean * 1e-3) / 1e9
+    out = {
+        'metric': 'survival-network hidden-layer product, HBM GB/s', 'value': achieved,
+        'unit': 'GB/s', 'n_gpus': 1, 'steps': steps, 'warmup': args.warmup,
+        'ms_per_step': wall / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'bf16 operands, f32 accumulate',
+        'data': 'synthetic (Gaussian operands)',
+        'config': {'workload': f'({M} x {K}) x ({K} x {N}) product with bias, ReLU and BF16 '
+                               f'output (inference of the 16-256-256-256-1 network over '
+                               f'{M} particles, hidden layer)',
+                   'kernel_ms': [round(v, 4) for v in ms], 'tflops': flop / (ms_mean * 1e-3) / 1e12,
+                   'training_step_ms_batch_1024': round(train_ms, 3)},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': 8000.0, 'unit': 'GB/s',
+                     'frac': achieved / 8000.0, 'traffic': None,
+                     'algorithmic_bytes_per_launch': bytes_, 'flop_per_launch': flop,
+                     'mfma_frac': flop / (ms_mean * 1e-3) / 2.5e15},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main_batch(args):
     """`--workload batch`: the batched-variants measurement as the line."""
     b = batch_leg(args, args.steps)
@@ -519,13 +597,21 @@ def main_da(args):
         ln.build_tracker(device='cuda:0')
     bt = xt.BatchTracker(lines, device='cuda:0', compile=True)
     log(f'da: {len(lines)} error-seed lattices compiled ({time.perf_counter() - t0:.1f} s)')
-    gs = surrogate.GridSet(lines, 20.0, args.da_amplitudes, args.da_angles, (0.0, 2.7e-4),
+    if args.da_grid_sizes:
+        lo, hi = (float(v) for v in args.da_grid_sizes.split(','))
+        rng = np.random.default_rng(777)
+        sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), len(lines)))
+        n_r = [max(2, int(round(v / (2 * args.da_angles)))) for v in sizes]
+    else:
+        n_r = args.da_amplitudes
+    gs = surrogate.GridSet(lines, 20.0, n_r, args.da_angles, (0.0, 2.7e-4),
                            NEMITT, NEMITT, torch.device('cuda:0'))
     norm = surrogate.normal_form_4d(lines[0], torch.device('cuda:0'))
     n = gs.n_slots
     log(f'da: grid of {n} particles built ({time.perf_counter() - t0:.1f} s)')
     T = args.da_turns
-    plain = surrogate.plain_da_scan_batch(bt, gs, T)
+    plain = surrogate.plain_da_scan_batch(
+        bt, gs, T, progress=lambda d, n_: log(f'da: plain, turn {d}, {n_} tracked'))
     log(f'da: plain scan {plain["time"]:.1f} s')
     out_scans = {}
     for pred in args.da_predictors.split(','):
@@ -536,7 +622,9 @@ def main_da(args):
                                          label_fraction=args.da_label_fraction,
                                          band_factor=3.0, band_sigma=args.da_band_sigma,
                                          epochs=args.da_epochs, seed=0, predictor=pred,
-                                         normalisation=norm)
+                                         normalisation=norm,
+                                         progress=lambda what, d, n_: log(
+                                             f'da: {pred} {what}, turn {d}, {n_} tracked'))
         wall = time.perf_counter() - t1
         ex = res['exact']
         assert np.array_equal(res['survival'][ex], plain['survival'][ex])
@@ -546,10 +634,16 @@ def main_da(args):
             'tracked_full_fraction': round(res['tracked_full_fraction'], 4),
             'parts_s': {k: round(v, 2) for k, v in res['time'].items()},
             'da_error_sigma_per_seed': [{k: round(v, 3) for k, v in e.items()} for e in errs],
-            'da_error_sigma_max': max(e['max_abs'] for e in errs)}
+            'da_error_sigma_max': max(e['max_abs'] for e in errs),
+            'da_error_sigma_seed_mean_max': max(abs(e['mean_diff']) for e in errs),
+            'da_error_sigma_seed_min_max': max(abs(e['min_diff']) for e in errs)}
         log(f'da: screened ({pred}) {wall:.1f} s, tracked {res["tracked_full_fraction"]:.3f}, '
-            f'max DA error {out_scans[pred]["da_error_sigma_max"]:.2f} sigma')
+            f'DA error: per angle max {out_scans[pred]["da_error_sigma_max"]:.2f}, '
+            f'per seed (mean over angles) max '
+            f'{out_scans[pred]["da_error_sigma_seed_mean_max"]:.3f}, per seed (minimum over '
+            f'angles) max {out_scans[pred]["da_error_sigma_seed_min_max"]:.3f} sigma')
     da_plain = [float(np.mean(list(d.values()))) for d in plain['da']]
+    da_plain_min = [float(min(d.values())) for d in plain['da']]
     out = {
         'metric': 'DA-scan wall time, plain vs surrogate-screened, 1 MI355X',
         'value': out_scans.get('mlp', next(iter(out_scans.values())))['speedup_vs_plain'],
@@ -559,10 +653,13 @@ def main_da(args):
         'data': f'synthetic (polar grid, collider ring with seeded b3..b6 errors of '
                 f'relative strength {args.da_error_scale:g})',
         'config': {'workload': f'DA scan of {len(lines)} error seeds of the collider ring, '
-                               f'{args.da_angles} angles x {args.da_amplitudes} amplitudes '
-                               f'(0-20 sigma) x 2 deltas per seed, {T} turns',
-                   'particles': n, 'plain_s': round(plain['time'], 2),
+                               f'{args.da_angles} angles x '
+                               f'{args.da_amplitudes if not args.da_grid_sizes else "uneven"} '
+                               f'amplitudes (0-20 sigma) x 2 deltas per seed, {T} turns',
+                   'particles': n, 'particles_per_seed': [int(v) for v in np.diff(gs.starts)],
+                   'plain_s': round(plain['time'], 2),
                    'plain_mean_da_sigma_per_seed': [round(v, 3) for v in da_plain],
+                   'plain_min_da_sigma_per_seed': [round(v, 3) for v in da_plain_min],
                    'screened': out_scans,
                    'feature_turns': args.da_feature_turns,
                    'label_fraction': args.da_label_fraction,
@@ -608,6 +705,9 @@ def main():
     if args.workload == 'da':
         main_da(args)
         return
+    if args.workload == 'mlp':
+        main_mlp(args)
+        return
     traffic = None
     if args.traffic == 'measure':
         try:
@@ -624,6 +724,9 @@ def main():
             lambda a, b: lattices.collider_ring(k2f=a, k2d=b))
 
     def make_line():
+        if args.ring_error_seed is not None:
+            return lattices.collider_with_errors(args.ring_error_seed, k2f=k2f, k2d=k2d,
+                                                 rel=args.ring_error_scale)
         return lattices.collider_ring(k2f=k2f, k2d=k2d)
 
     line = make_line()
@@ -683,7 +786,10 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic (seeded matched Gaussian beam, synthetic lattice)',
         'config': {'workload': 'collider ring, 8 arcs x 23 FODO cells, '
-                               f'{len(line.elements)} elements, 7 TeV protons',
+                               f'{len(line.elements)} elements, 7 TeV protons' +
+                               ('' if args.ring_error_seed is None else
+                                f', b3..b6 errors of seed {args.ring_error_seed} '
+                                f'(relative {args.ring_error_scale:g})'),
                    'particles': args.particles, 'turns_per_step': tps,
                    'kernel': 'compiled line code' if compiled else 'element-program interpreter',
                    'particles_alive_end': n_alive,

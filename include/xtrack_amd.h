@@ -245,17 +245,21 @@ typedef struct xtm_monitor {
  * z = inv_w (x - co_x, px - co_px, y - co_y, py - co_py) scaled by
  * inv_sigma (normalised coordinates), jx = (zx^2 + zpx^2) / 2, jy likewise,
  * and the phase advance since the previous recorded turn in each plane
- * (atan2 of the rotation between consecutive (z, zp), in [0, 2 pi)).
+ * (atan2 of the rotation between consecutive (z, zp), in [0, 2 pi)), summed
+ * over all recorded turns and, separately, over the turns of the second half
+ * of the window (at_turn >= (start_at_turn + stop_at_turn) / 2): the tunes of
+ * the two halves, whose difference is the tune diffusion.
  * acc holds XTM_FEAT_N x capacity doubles, accumulator k of particle slot i
  * at acc[k * capacity + i]; zero it before the first recorded turn. */
-#define XTM_FEAT_N 15
+#define XTM_FEAT_N 17
 enum {
     XTM_FEAT_COUNT = 0,      /* turns recorded */
     XTM_FEAT_JX_SUM = 1, XTM_FEAT_JX_SQ = 2, XTM_FEAT_JX_MAX = 3,
     XTM_FEAT_JY_SUM = 4, XTM_FEAT_JY_SQ = 5, XTM_FEAT_JY_MAX = 6,
     XTM_FEAT_PHX_SUM = 7, XTM_FEAT_PHY_SUM = 8,
     XTM_FEAT_ZX = 9, XTM_FEAT_ZPX = 10, XTM_FEAT_ZY = 11, XTM_FEAT_ZPY = 12,   /* last turn */
-    XTM_FEAT_JX0 = 13, XTM_FEAT_JY0 = 14
+    XTM_FEAT_JX0 = 13, XTM_FEAT_JY0 = 14,
+    XTM_FEAT_PHX_H2 = 15, XTM_FEAT_PHY_H2 = 16   /* phase sums, second half */
 };
 typedef struct xtm_features {
     double *acc;
@@ -402,11 +406,12 @@ int32_t xtm_batch_destroy(xtm_batch *batch);
 
 /* Particles of line k are [line_start[k], line_start[k+1]) of the particle
  * set passed to xtm_batch_track (line_start has num_lines + 1 entries,
- * non-decreasing, from 0).  block_size (256 threads: the per-thread LDS
- * slots of the kernels are laid out for it) and waves_per_eu (3, 4 or 5,
- * the occupancy the kernel instance was compiled for) select the kernel
- * instance.  Builds the block table: each block holds particles of one
- * line only, and blocks of longer lines come first. */
+ * non-decreasing, from 0).  block_size (64, 128 or 256 threads; the
+ * per-thread LDS slots are sized by the launch) and waves_per_eu (3, 4 or
+ * 5, the occupancy of the interpreter kernel instance; the compiled batch
+ * code is built for 5) select the launch.  Builds the block table: each
+ * block holds particles of one line only, and blocks of longer lines come
+ * first. */
 int32_t xtm_batch_set_layout(xtm_batch *batch, const int64_t *line_start,
                              int32_t block_size, int32_t waves_per_eu);
 

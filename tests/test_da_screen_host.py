@@ -85,3 +85,46 @@ def test_predicted_aperture_uses_half_the_turns():
     band = S.boundary_band(gs, guess, rest, T, band_factor=1.0, band_sigma=0.5)
     r = gs.r_slot
     assert np.allclose(sorted(r[band]), sorted(r[np.abs(r - r[30]) <= 0.5]))
+
+
+def test_below_first_loss_marks_the_slots_that_can_change_the_aperture():
+    """Per (lattice, angle, delta) group: the slots below the smallest lost
+    amplitude; a group without losses keeps every slot; the aperture of the
+    survival turns is unchanged when only those slots are known."""
+    T = 1000
+    gs = _grid(n_r=(40, 55))
+    da_of = lambda l, th: 4.0 + l + 2 * th
+    surv, ap = _truth(gs, T, da_of)
+    lost = surv < T
+    below = S.below_first_loss(gs, lost)
+    assert not np.any(below & lost)
+    assert np.all(gs.r_slot[below] <= ap[below] + 1e-12)
+    # slots above the first loss carry no information on the aperture
+    masked = surv.copy()
+    masked[~below & ~lost] = 1
+    for l in range(gs.n_lines):
+        sl = slice(gs.starts[l], gs.starts[l + 1])
+        assert da.dynamic_aperture(gs.r_slot[sl], gs.theta_slot[sl], gs.delta_slot[sl],
+                                   masked[sl], T) == \
+            da.dynamic_aperture(gs.r_slot[sl], gs.theta_slot[sl], gs.delta_slot[sl],
+                                surv[sl], T)
+    # no loss at all: every slot counts
+    assert np.all(S.below_first_loss(gs, np.zeros(gs.n_slots, bool)))
+
+
+def test_band_uses_the_ensemble_range():
+    """A slot is in the factor band when any ensemble member's prediction
+    falls near the target; the predicted aperture uses the smallest
+    prediction."""
+    T = 1000
+    gs = _grid(n_lines=1, deltas=(0.0,), n_angles=1)
+    rest = np.arange(gs.n_slots)
+    guess = np.full(gs.n_slots, 10.0 * T)
+    lo, hi = guess.copy(), guess.copy()
+    lo[5], hi[5] = 2000.0, 8000.0        # one member within a factor 3
+    lo[7], hi[7] = 5000.0, 9000.0        # none
+    band = S.boundary_band(gs, guess, rest, T, 3.0, -1.0, guess_lo=lo, guess_hi=hi)
+    assert 5 in band and 7 not in band
+    lo[20] = 100.0                       # one member predicts a loss at slot 20
+    band = S.boundary_band(gs, guess, rest, T, 1.0, 0.26, guess_lo=lo, guess_hi=hi)
+    assert set(band.tolist()) == {k for k in rest if abs(gs.r_slot[k] - gs.r_slot[20]) <= 0.26}

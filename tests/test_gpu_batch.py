@@ -110,16 +110,18 @@ def test_batch_block_sizes_bit_exact(block):
             assert not bad, (k, bad)
 
 
-def test_batch_graph_replays_match_eager():
+@pytest.mark.parametrize('compiled', [False, True], ids=['interpreter', 'compiled'])
+def test_batch_graph_replays_match_eager(compiled):
     """One launch of 2 turns captured in a HIP graph and replayed three
     times, with eager work and synchronisation between the replays; after
     each replay the particles equal those of the same number of eager
     launches."""
     import torch
-    print('[batch] graph replays', flush=True)
+    print(f'[batch] graph replays, compiled={compiled}', flush=True)
     lines = _variants()
     parts = _same_species(_particles())
-    bt = xt.BatchTracker(lines, device='cuda:0')
+    bt = xt.BatchTracker(lines, device='cuda:0', compile=compiled)
+    assert bt.is_compiled == compiled
     big, starts = bt._concat(parts)
     bt._set_layout(starts, (256, 4))
     eager = big.copy()
@@ -155,7 +157,7 @@ def test_batch_tuner_kernel_instances_agree():
     print('[batch] tuner', flush=True)
     lines = _variants()
     parts = _same_species(_particles())
-    bt = xt.BatchTracker(lines, device='cuda:0')
+    bt = xt.BatchTracker(lines, device='cuda:0', compile=False)
     cfg = bt.tune(parts, turns=1)
     assert cfg in xt.batch.CANDIDATES and set(bt.tune_timings) == set(xt.batch.CANDIDATES)
     assert {c[0] for c in xt.batch.CANDIDATES} == {64, 128, 256}

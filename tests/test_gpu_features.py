@@ -42,6 +42,10 @@ def _expected(mon, inv_w4, co, sx, sy, n_turns):
     a2 = alive[:, 1:]
     out['phx_sum'] = (ph(zx, zpx) * a2).sum(1)
     out['phy_sum'] = (ph(zy, zpy) * a2).sum(1)
+    # second half of the window: advances recorded at turns t >= n_turns / 2
+    h2 = 2 * np.arange(1, n_turns) >= n_turns
+    out['phx_h2'] = (ph(zx, zpx) * a2 * h2).sum(1)
+    out['phy_h2'] = (ph(zy, zpy) * a2 * h2).sum(1)
     return out
 
 
@@ -74,7 +78,7 @@ def test_line_features_match_monitor(compile):
     a = acc[:, order]
     assert np.array_equal(a[0], exp["count"])
     for k, key in ((1, 'jx_sum'), (4, 'jy_sum'), (3, 'jx_max'), (13, 'jx0'), (14, 'jy0'),
-                   (7, 'phx_sum'), (8, 'phy_sum')):
+                   (7, 'phx_sum'), (8, 'phy_sum'), (15, 'phx_h2'), (16, 'phy_h2')):
         np.testing.assert_allclose(a[k], exp[key], rtol=1e-11, atol=1e-12, err_msg=key)
 
 

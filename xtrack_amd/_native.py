@@ -40,7 +40,7 @@ class XtmMonitor(ctypes.Structure):
                 ('data', XtmParticles)]
 
 
-XTM_FEAT_N = 15
+XTM_FEAT_N = 17
 
 
 class XtmFeatures(ctypes.Structure):

@@ -47,7 +47,7 @@ COMPILED_CANDIDATES = tuple((bs, 5) for bs in BLOCK_SIZES)
 
 
 class BatchTracker:
-    def __init__(self, lines, device='cuda:0', compile=False):
+    def __init__(self, lines, device='cuda:0', compile='auto'):
         import torch
         self.lib = _native.load()
         if self.lib.xtm_device_count() < 1:

@@ -1344,6 +1344,10 @@ XT_HD void feature_record(const xtm_features &F, const int64_t i, const double x
         if (py_ < 0.0) py_ += twopi;
         a[XTM_FEAT_PHX_SUM * c + i] += px_;
         a[XTM_FEAT_PHY_SUM * c + i] += py_;
+        if (2 * at_turn >= F.start_at_turn + F.stop_at_turn) {
+            a[XTM_FEAT_PHX_H2 * c + i] += px_;
+            a[XTM_FEAT_PHY_H2 * c + i] += py_;
+        }
         double &mx = a[XTM_FEAT_JX_MAX * c + i];
         double &my = a[XTM_FEAT_JY_MAX * c + i];
         if (jx > mx) mx = jx;

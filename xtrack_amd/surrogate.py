@@ -8,12 +8,16 @@ matrix cores through xtm_gemm_bf16 (BF16 operands, FP32 accumulation),
 forward and backward; the FP32 master weights, the bias and loss sums and
 the Adam update are torch tensor operations on the same device.
 
-The scan (``surrogate_da_scan``) tracks every grid particle for the feature
-turns, trains the network on a tracked subset, and tracks to the full turn
-count only the particles whose predicted survival lies within a factor of
-the target (the boundary band); particles the network places well inside
-or outside are assigned their prediction.  Plain full tracking of the same
-grid is the reference the scan's aperture error is measured against.
+The screened scan (``screened_da_scan``, over many lattices at once on the
+batch tracker) tracks every grid particle for the feature turns with the
+feature accumulators updated on the device, tracks a random subset of the
+survivors to the full turn count for labels, fits an ensemble of networks
+to them, and tracks to the full turn count only the particles whose
+predicted survival lies within a factor of the target or whose amplitude
+lies near the predicted aperture (the boundary band); particles the
+networks place well inside or outside are assigned their prediction.
+Plain full tracking of the same grid is the reference the scan's aperture
+error is measured against.
 """
 import ctypes
 import math
@@ -244,12 +248,15 @@ def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, star
     return acc
 
 
-def features_from_acc(acc, x_norm0, y_norm0, delta):
-    """The 16 network inputs per particle from the in-kernel accumulators:
-    r, theta, delta, the initial actions, mean, spread and maximum of the
-    normalised actions over the recorded turns (relative to the initial
-    ones), the mean phase advance per turn in each plane (tunes), the
-    fraction of the turns recorded alive, cos and sin of theta."""
+def features_from_acc(acc, x_norm0, y_norm0, delta, window):
+    """The 16 network inputs per particle from the in-kernel accumulators
+    of a feature window of `window` turns: r, theta, delta, the initial
+    actions, mean, spread and maximum of the normalised actions over the
+    recorded turns (relative to the initial ones), the mean phase advance
+    per turn in each plane (tunes), the fraction of the turns recorded
+    alive, and log10 of the tune diffusion in each plane (the difference of
+    the mean phase advances of the window's second and first halves, as in
+    frequency-map analysis)."""
     torch = _t()
     dev = acc.device
     n = acc[0].clamp_min(1.0)       # turns recorded
@@ -260,6 +267,15 @@ def features_from_acc(acc, x_norm0, y_norm0, delta):
     j0x, j0y = acc[13].clamp_min(1e-12), acc[14].clamp_min(1e-12)
     nph = (acc[0] - 1.0).clamp_min(1.0)
     qx, qy = acc[7] / nph / (2 * math.pi), acc[8] / nph / (2 * math.pi)
+    # advances recorded at turns >= start + ceil(window / 2): the second half
+    n2 = (acc[0] - max((int(window) + 1) // 2, 1)).clamp_min(0.0)
+    n1 = (acc[0] - 1.0 - n2).clamp_min(0.0)
+    ph1x, ph1y = acc[7] - acc[15], acc[8] - acc[16]
+    dqx = (acc[15] / n2.clamp_min(1.0) - ph1x / n1.clamp_min(1.0)).abs() / (2 * math.pi)
+    dqy = (acc[16] / n2.clamp_min(1.0) - ph1y / n1.clamp_min(1.0)).abs() / (2 * math.pi)
+    both = (n1 > 0) & (n2 > 0)
+    ldqx = torch.where(both, torch.log10(dqx + 1e-12), torch.zeros_like(dqx))
+    ldqy = torch.where(both, torch.log10(dqy + 1e-12), torch.zeros_like(dqy))
     r0 = torch.sqrt(torch.as_tensor(x_norm0 ** 2 + y_norm0 ** 2, device=dev))
     th0 = torch.atan2(torch.as_tensor(y_norm0, device=dev), torch.as_tensor(x_norm0, device=dev))
     f = torch.stack([r0, th0, torch.as_tensor(delta * 1e3, device=dev),
@@ -268,7 +284,7 @@ def features_from_acc(acc, x_norm0, y_norm0, delta):
                      sjx / mjx.clamp_min(1e-12), sjy / mjy.clamp_min(1e-12),
                      torch.log(acc[3].clamp_min(1e-12) / j0x),
                      torch.log(acc[6].clamp_min(1e-12) / j0y),
-                     qx, qy, acc[0] / nturn, torch.cos(th0), torch.sin(th0)], dim=1)
+                     qx, qy, acc[0] / nturn, ldqx, ldqy], dim=1)
     return torch.nan_to_num(f, nan=0.0, posinf=50.0, neginf=-50.0).to(torch.float32)
 
 
@@ -335,13 +351,14 @@ def _starts_of(slots_sorted, starts):
     return np.searchsorted(slots_sorted, starts).astype(np.int64)
 
 
-def track_subset(bt, gs, slots, num_turns, block=None, features=None):
+def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=None):
     """Track the grid particles `slots` (sorted numpy slot indices) of
     GridSet gs for num_turns turns on BatchTracker bt, compacting the lost
     ones out every `block` turns (all turns in one launch when None), and
     write the results back into gs.particles.  features: None, or
     (inv_w4, co) to gather the feature accumulators (with block None);
-    returns them for the slots, else None."""
+    returns them for the slots, else None.  progress: None, or called as
+    progress(turns done, particles still tracked) after each block."""
     torch = _t()
     dev = gs.device
     slots = np.asarray(slots, dtype=np.int64)
@@ -362,6 +379,8 @@ def track_subset(bt, gs, slots, num_turns, block=None, features=None):
         if done < num_turns:
             alive = (sub._arrays['state'] > 0).cpu().numpy()
             slots = slots[alive]
+        if progress is not None:
+            progress(done, len(slots))
     torch.cuda.synchronize(dev)
     return acc
 
@@ -384,14 +403,14 @@ def da_per_line(gs, surv, num_turns):
     return out
 
 
-def plain_da_scan_batch(bt, gs, num_turns, block=256):
+def plain_da_scan_batch(bt, gs, num_turns, block=256, progress=None):
     """Every grid particle of every lattice tracked to num_turns (lost
     particles compacted out every `block` turns).  Returns survival turns,
     the apertures per lattice and the wall time."""
     torch = _t()
     torch.cuda.synchronize(gs.device)
     t0 = time.perf_counter()
-    track_subset(bt, gs, np.arange(gs.n_slots), num_turns, block=block)
+    track_subset(bt, gs, np.arange(gs.n_slots), num_turns, block=block, progress=progress)
     wall = time.perf_counter() - t0
     surv = survival_of(gs, num_turns)
     return {'survival': surv, 'da': da_per_line(gs, surv, num_turns), 'time': wall}
@@ -432,24 +451,47 @@ def interp_predictor(gs, labelled, surv, candidates):
     return pred
 
 
-def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_below=0.5):
+def _group_id(gs):
+    """Per slot: the index of its (lattice, angle, delta) group."""
+    if getattr(gs, '_gid', None) is None:
+        keys = np.stack(_group_keys(gs, np.arange(gs.n_slots)), axis=1)
+        gs._gid = np.unique(keys, axis=0, return_inverse=True)[1].reshape(-1)
+    return gs._gid
+
+
+def below_first_loss(gs, lost):
+    """Per slot: True when its amplitude lies below the smallest amplitude
+    of a particle of `lost` (bool per slot) in its (lattice, angle, delta)
+    group.  Only these slots can still change the group's aperture (the
+    largest amplitude below which every particle survived)."""
+    gid = _group_id(gs)
+    first = np.full(int(gid.max()) + 1 if len(gid) else 0, np.inf)
+    np.minimum.at(first, gid[lost], gs.r_slot[lost])
+    return gs.r_slot < first[gid]
+
+
+def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_below=0.5,
+                  guess_lo=None, guess_hi=None):
     """The slots of `rest` to track in full.  guess: survival per slot,
-    exact where known, predicted for the slots of `rest`.  A slot of `rest`
-    is in the band when its predicted survival lies within a factor
-    band_factor of num_turns, or its amplitude within band_sigma of the
-    predicted aperture of its (lattice, angle, delta) group: the smallest
-    amplitude guessed lost (known survival below num_turns, or predicted
-    survival below lost_below * num_turns; the largest amplitude when none
-    is)."""
+    exact where known, predicted for the slots of `rest`; guess_lo and
+    guess_hi: the smallest and largest prediction of an ensemble (default
+    guess).  A slot of `rest` is in the band when its predicted survival
+    range meets [num_turns / band_factor, num_turns * band_factor], or its
+    amplitude lies within band_sigma of the predicted aperture of its
+    (lattice, angle, delta) group: the smallest amplitude guessed lost
+    (known survival below num_turns, or smallest prediction below
+    lost_below * num_turns; the largest amplitude when none is)."""
     rest = np.asarray(rest, dtype=np.int64)
-    pred = guess[rest]
-    in_band = (pred >= num_turns / band_factor) & (pred <= num_turns * band_factor)
+    lo_all = guess if guess_lo is None else guess_lo
+    hi_all = guess if guess_hi is None else guess_hi
+    lo, hi = lo_all[rest], hi_all[rest]
+    in_band = (hi >= num_turns / band_factor) & (lo <= num_turns * band_factor)
     n_all = len(guess)
     r_all = gs.r_slot
     keys = np.stack(_group_keys(gs, np.arange(n_all)), axis=1)
     gid = np.unique(keys, axis=0, return_inverse=True)[1].reshape(-1)
     lost = guess < num_turns
-    lost[rest] = pred < lost_below * num_turns
+    lost[rest] = lo < lost_below * num_turns
     lost_r = np.full(int(gid.max()) + 1, np.inf)
     np.minimum.at(lost_r, gid[lost], r_all[lost])
     r_pred = np.where(np.isfinite(lost_r), lost_r, gs.r_max)
@@ -459,24 +501,34 @@ def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_belo
 
 def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
                      band_factor=3.0, band_sigma=0.5, epochs=150, seed=0, net=None,
-                     predictor='mlp', block=256, normalisation=None):
+                     predictor='mlp', block=256, normalisation=None, censor_margin=None,
+                     ensemble=4, progress=None):
     """Dynamic-aperture scan of every lattice of GridSet gs screened by the
     survival surrogate, on BatchTracker bt (the lattices of gs, in order):
 
     1. every grid particle is tracked for feature_turns turns with the
        feature accumulators gathered in the kernel;
-    2. a random label_fraction of the survivors of each lattice is tracked
-       to num_turns: their survival turns are the labels;
+    2. a random label_fraction of the survivors of each lattice that lie
+       below the first loss of their (angle, delta) group (only those can
+       change the aperture) is tracked to num_turns: their survival turns
+       are the labels;
     3. the network (16-256-256-256-1, BF16 products on the matrix cores) is
-       fit to log survival turn on the labels, the survivors' turn count
-       taken as a lower bound (censored loss) (predictor='interp': no
-       network, per-angle interpolation of the labels in amplitude);
-    4. the survivors whose predicted survival lies within a factor
-       band_factor of num_turns, or whose amplitude lies within band_sigma
-       beam sigmas of their angle's predicted aperture, are tracked to
+       fit to log survival turn on the labels; for a label alive at
+       num_turns the target is a lower bound, censor_margin x num_turns
+       (default 2), counted only where the output lies below it, so that
+       the network may place stable particles beyond the band; `ensemble`
+       networks from different initialisations and minibatch orders are
+       fit (predictor='interp': no network, per-angle interpolation of the
+       labels in amplitude);
+    4. of the untracked survivors below the first known loss of their
+       group, those for which any network predicts a survival within a
+       factor band_factor of num_turns, or whose amplitude lies within
+       band_sigma beam sigmas of their angle's predicted aperture (the
+       smallest amplitude that any network predicts lost), are tracked to
        num_turns (the boundary band);
-    5. every other particle keeps its prediction: alive if predicted to
-       survive at least half of num_turns, else lost at its predicted turn.
+    5. every other particle keeps its prediction (the ensemble's median):
+       alive if predicted to survive at least half of num_turns, else lost
+       at its predicted turn.
 
     Tracking is continued, not restarted: a particle tracked in full has
     exactly its plain-scan survival turn.  normalisation: (inv_w4, co) of
@@ -492,12 +544,14 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     acc = track_subset(bt, gs, np.arange(n_all), feature_turns, features=normalisation)
-    feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_slot)
+    feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_slot, feature_turns)
     torch.cuda.synchronize(dev)
     times['features'] = time.perf_counter() - t0
     surv = survival_of(gs, feature_turns)
     exact = surv < feature_turns
-    alive = np.flatnonzero(~exact)
+    # particles above a loss of their group cannot change its aperture: the
+    # labels and the band are drawn from the survivors below the first loss
+    alive = np.flatnonzero(~exact & below_first_loss(gs, exact))
     # 2. labels
     rng = np.random.default_rng(seed)
     lab = []
@@ -508,7 +562,9 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
             lab.append(np.sort(rng.choice(al, size=k, replace=False)))
     lab = np.concatenate(lab) if lab else np.zeros(0, np.int64)
     t0 = time.perf_counter()
-    track_subset(bt, gs, lab, num_turns - feature_turns, block=block)
+    track_subset(bt, gs, lab, num_turns - feature_turns, block=block,
+                 progress=None if progress is None else
+                 (lambda d, n: progress('labels', feature_turns + d, n)))
     times['label_tracking'] = time.perf_counter() - t0
     surv = survival_of(gs, num_turns)
     exact[lab] = True
@@ -516,18 +572,30 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     t0 = time.perf_counter()
     rest = np.flatnonzero(~exact)
     hist = None
+    nets = []
     if predictor == 'mlp':
-        net = net or MLP(device=dev, seed=seed)
         fx = feat[torch.as_tensor(lab, device=dev)]
-        fy = torch.as_tensor(np.log(surv[lab].astype(np.float64)), dtype=torch.float32,
-                             device=dev)
-        cens = torch.as_tensor(surv[lab] >= num_turns, device=dev)
-        hist = net.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(lab) // 4)),
-                       seed=seed, censored=cens)
-        pred = np.exp(np.minimum(net.predict(feat[torch.as_tensor(rest, device=dev)])
-                                 .cpu().numpy().astype(np.float64), math.log(1e12)))
+        alive_lab = surv[lab] >= num_turns
+        margin = 2.0 if censor_margin is None else float(censor_margin)
+        fy = torch.as_tensor(np.log(np.where(alive_lab, margin * num_turns,
+                                             surv[lab].astype(np.float64))),
+                             dtype=torch.float32, device=dev)
+        cens = torch.as_tensor(alive_lab, device=dev)
+        fr = feat[torch.as_tensor(rest, device=dev)]
+        preds = []
+        for k in range(int(ensemble)):
+            nk = net if (k == 0 and net is not None) else MLP(device=dev, seed=seed + k)
+            h = nk.fit(fx, fy, epochs=epochs, batch=min(1024, max(64, len(lab) // 4)),
+                       seed=seed + k, censored=cens)
+            hist = h if hist is None else hist
+            preds.append(np.exp(np.minimum(nk.predict(fr).cpu().numpy().astype(np.float64),
+                                           math.log(1e12))))
+            nets.append(nk)
+        P = np.stack(preds)
+        pred, p_lo, p_hi = np.median(P, axis=0), P.min(axis=0), P.max(axis=0)
     elif predictor == 'interp':
         pred = interp_predictor(gs, lab, surv, rest)
+        p_lo = p_hi = pred
     else:
         raise ValueError("predictor must be 'mlp' or 'interp'")
     torch.cuda.synchronize(dev)
@@ -536,16 +604,21 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     # the predicted aperture of its (lattice, angle, delta) group
     guess = surv.astype(np.float64)
     guess[rest] = pred
-    band = boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma)
+    lo, hi = guess.copy(), guess.copy()
+    lo[rest], hi[rest] = p_lo, p_hi
+    relevant = rest[below_first_loss(gs, exact & (surv < num_turns))[rest]]
+    band = boundary_band(gs, guess, relevant, num_turns, band_factor, band_sigma,
+                         guess_lo=lo, guess_hi=hi)
     t0 = time.perf_counter()
-    track_subset(bt, gs, band, num_turns - feature_turns, block=block)
+    track_subset(bt, gs, band, num_turns - feature_turns, block=block,
+                 progress=None if progress is None else
+                 (lambda d, n: progress('band', feature_turns + d, n)))
     times['band_tracking'] = time.perf_counter() - t0
     surv_t = survival_of(gs, num_turns)
     surv[band] = surv_t[band]
     exact[band] = True
-    # 5. the rest keeps its prediction: alive when predicted to survive half
-    # the turns or more (the network regresses log survival, capped at
-    # num_turns for the survivors, so their predictions scatter around it)
+    # 5. the rest keeps its prediction (the ensemble's median): alive when
+    # predicted to survive half the turns or more
     left = np.flatnonzero(~exact)
     p_left = guess[left]
     surv[left] = np.where(p_left >= 0.5 * num_turns, num_turns,
@@ -554,7 +627,7 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     return {'survival': surv, 'exact': exact, 'labelled': lab, 'band': band,
             'tracked_full_fraction': (len(lab) + len(band)) / n_all,
             'da': da_per_line(gs, surv, num_turns), 'loss_history': hist, 'time': times,
-            'net': net if predictor == 'mlp' else None}
+            'net': nets[0] if nets else None, 'nets': nets}
 
 
 def normal_form_4d(line, device):
@@ -572,13 +645,15 @@ def normal_form_4d(line, device):
 
 
 def da_errors(da_a, da_b):
-    """Per lattice: the largest |difference| of the per-angle apertures and
-    the difference of their means (beam sigmas)."""
+    """Per lattice: the largest |difference| of the per-angle apertures,
+    the difference of their means and the difference of their minima (the
+    lattice's aperture over all angles and deltas), in beam sigmas."""
     out = []
     for a, b in zip(da_a, da_b):
         d = np.array([a[k] - b[k] for k in b])
         out.append({'max_abs': float(np.max(np.abs(d))),
-                    'mean_diff': float(np.mean([a[k] for k in b]) - np.mean(list(b.values())))})
+                    'mean_diff': float(np.mean([a[k] for k in b]) - np.mean(list(b.values()))),
+                    'min_diff': float(min(a[k] for k in b) - min(b.values()))})
     return out
 
 
@@ -596,7 +671,7 @@ def surrogate_da_scan(line, r_max, n_r, n_angles, num_turns, nemitt_x, nemitt_y,
     gs = GridSet([line], r_max, n_r, n_angles, deltas, nemitt_x, nemitt_y, dev)
     res = screened_da_scan(bt, gs, num_turns, feature_turns=feature_turns,
                            label_fraction=train_fraction, band_factor=1.0, band_sigma=band,
-                           epochs=epochs, seed=seed, net=net)
+                           epochs=epochs, seed=seed, net=net, ensemble=1)
     return {'r': gs.r_slot, 'theta': gs.theta_slot, 'delta': gs.delta_slot,
             'survival': res['survival'],
             'exact': res['exact'], 'tracked_full': np.union1d(res['labelled'], res['band']),
