@@ -121,6 +121,8 @@ def parse():
     ap.add_argument('--batch-compile', action='store_true',
                     help='compile the variants into one kernel (element structures with '
                          'constants read from each line) instead of the interpreter')
+    ap.add_argument('--batch-waves-per-eu', type=int, default=0,
+                    help='occupancy target of the compiled batch code (0: library default)')
     return ap.parse_args()
 
 
@@ -427,7 +429,8 @@ def batch_leg(args, steps, kernels=None):
     flop = None
     for kern in kernels:
         t1 = time.perf_counter()
-        bt = xt.BatchTracker(lines, device='cuda:0', compile=(kern == 'compiled'))
+        bt = xt.BatchTracker(lines, device='cuda:0', compile=(kern == 'compiled'),
+                             waves_per_eu=args.batch_waves_per_eu)
         setup_s = time.perf_counter() - t1
         log(f'batch ({kern}): tracker ready ({setup_s:.1f} s)')
         if flop is None:

@@ -26,7 +26,8 @@ candidate on a copy of the particles and keeps the fastest: block sizes 64,
 each variant and spread a small particle set over more compute units),
 occupancy 3, 4 or 5 waves per SIMD for the interpreter kernel.  The
 compiled batch code is built for 5 waves per SIMD, so for a compiled batch
-only the block size is searched.  The per-thread LDS slots of the particle
+only the block size is searched (``waves_per_eu`` sets the compiled code's
+occupancy target, 0 for the library default).  The per-thread LDS slots of the particle
 state are sized by the launch (block size x 48 bytes), so every block size
 runs the same code.
 """
@@ -47,7 +48,7 @@ COMPILED_CANDIDATES = tuple((bs, 5) for bs in BLOCK_SIZES)
 
 
 class BatchTracker:
-    def __init__(self, lines, device='cuda:0', compile='auto'):
+    def __init__(self, lines, device='cuda:0', compile='auto', waves_per_eu=0):
         import torch
         self.lib = _native.load()
         if self.lib.xtm_device_count() < 1:
@@ -75,7 +76,7 @@ class BatchTracker:
         if compile not in ('auto', True, False):
             raise ValueError("compile must be 'auto', True or False")
         if compile is not False:
-            st = self.lib.xtm_batch_specialize(self._batch, 0)
+            st = self.lib.xtm_batch_specialize(self._batch, int(waves_per_eu))
             # too many distinct element structures: 'auto' keeps the interpreter
             if st != 0 and not (st == _native.XTM_ERR_UNSUPPORTED and compile == 'auto'):
                 _native.check(st)

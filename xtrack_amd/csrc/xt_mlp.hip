@@ -11,9 +11,13 @@
 //
 // Each 256-thread block computes a 64 x 64 tile of C over a range of k: the
 // four waves stage 64 x 32 tiles of op(A) and op(B) in LDS (k contiguous
-// per row, so that a lane's fragment is one 16-byte LDS read), then each
-// wave multiplies its 32 x 32 quarter as 2 x 2 tiles of gfx950's
-// v_mfma_f32_16x16x32_bf16.  Operand layout of that instruction (64 lanes,
+// per row, so that a lane's fragment is one 16-byte LDS read; 16-byte
+// global loads where the operand allows, the next k step's tiles loaded
+// into registers while this step's products run), then each wave
+// multiplies its 32 x 32 quarter as 2 x 2 tiles of gfx950's
+// v_mfma_f32_16x16x32_bf16.  The result tile goes back through LDS so
+// that each thread writes 16 consecutive columns of one row with vector
+// stores (FP32, and the BF16 copy).  Operand layout of that instruction (64 lanes,
 // lane l): A holds row l % 16 and the eight consecutive k = 8 (l / 16) + j;
 // B holds column l % 16 and the same k; the accumulator holds column l % 16
 // and rows 4 (l / 16) + j, j < 4.
@@ -38,9 +42,11 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int TM = 64, TN = 64, TK = 32;
 constexpr int LDK = TK + 8;     // LDS row: 40 bf16 = 80 bytes (16-byte aligned fragments)
+constexpr int CLD = TN + 4;     // LDS row of the result tile: 68 floats (16-byte aligned)
 constexpr int64_t GEMM_WS_FLOATS = 1 << 20;   // split partials: splits x M x N <= 256 x 64 x 64
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -52,32 +58,52 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
     return (uint16_t)(u >> 16);
 }
 
-/* op(X) tile rows r0.., k k0.. into S[row][k] (bf16 bits, zero outside
-   [0, R) x [0, Kend)); X(r, k) = X[r * sr + k * sk].  256 threads. */
-__device__ __forceinline__ void stage(uint16_t *S, const uint16_t *__restrict__ X, int64_t sr,
-                                      int64_t sk, int64_t r0, int64_t k0, int64_t R,
-                                      int64_t Kend, int tid) {
+/* A thread's share of the op(X) tile rows r0.., k k0.. (bf16 bits, zero
+   outside [0, R) x [0, Kend)); X(r, k) = X[r * sr + k * sk].  256 threads,
+   8 elements each: with k contiguous (sk == 1) 8 consecutive k of one row,
+   with rows contiguous (sr == 1) 8 consecutive rows of one k, else 8
+   elements 256 apart in the tile.  Contiguous elements inside the operand
+   that start on a 16-byte boundary are one 16-byte load. */
+__device__ __forceinline__ u16x8 load_tile(const uint16_t *__restrict__ X, int64_t sr,
+                                           int64_t sk, int64_t r0, int64_t k0, int64_t R,
+                                           int64_t Kend, int tid) {
+    const bool base16 = ((uintptr_t)X & 15) == 0;
+    u16x8 v;
     if (sk == 1) {
-        // k contiguous: 8 consecutive k of one row per thread
         const int r = tid >> 2, c = (tid & 3) * 8;
-        const int64_t row = r0 + r;
-        for (int j = 0; j < 8; j++) {
-            const int64_t k = k0 + c + j;
-            S[r * LDK + c + j] = (row < R && k < Kend) ? X[row * sr + k] : (uint16_t)0;
-        }
+        const int64_t row = r0 + r, k = k0 + c, off = row * sr + k;
+        if (base16 && row < R && k + 8 <= Kend && (off & 7) == 0)
+            return *reinterpret_cast<const u16x8 *>(&X[off]);
+        for (int j = 0; j < 8; j++) v[j] = (row < R && k + j < Kend) ? X[off + j] : (uint16_t)0;
     } else if (sr == 1) {
-        // rows contiguous: 8 consecutive rows of one k per thread
         const int kk = tid >> 3, r = (tid & 7) * 8;
-        const int64_t k = k0 + kk;
-        for (int j = 0; j < 8; j++) {
-            const int64_t row = r0 + r + j;
-            S[(r + j) * LDK + kk] = (row < R && k < Kend) ? X[row + k * sk] : (uint16_t)0;
-        }
+        const int64_t k = k0 + kk, row = r0 + r, off = row + k * sk;
+        if (base16 && k < Kend && row + 8 <= R && (off & 7) == 0)
+            return *reinterpret_cast<const u16x8 *>(&X[off]);
+        for (int j = 0; j < 8; j++) v[j] = (row + j < R && k < Kend) ? X[off + j] : (uint16_t)0;
     } else {
-        for (int e = tid; e < TM * TK; e += 256) {
-            const int r = e / TK, c = e % TK;
+        for (int j = 0; j < 8; j++) {
+            const int e = tid + 256 * j, r = e / TK, c = e % TK;
             const int64_t row = r0 + r, k = k0 + c;
-            S[r * LDK + c] = (row < R && k < Kend) ? X[row * sr + k * sk] : (uint16_t)0;
+            v[j] = (row < R && k < Kend) ? X[row * sr + k * sk] : (uint16_t)0;
+        }
+    }
+    return v;
+}
+
+/* The thread's share (load_tile) into the LDS tile S[row][k]. */
+__device__ __forceinline__ void store_tile(uint16_t *S, const u16x8 v, int64_t sr, int64_t sk,
+                                           int tid) {
+    if (sk == 1) {
+        const int r = tid >> 2, c = (tid & 3) * 8;
+        *reinterpret_cast<u16x8 *>(&S[r * LDK + c]) = v;
+    } else if (sr == 1) {
+        const int kk = tid >> 3, r = (tid & 7) * 8;
+        for (int j = 0; j < 8; j++) S[(r + j) * LDK + kk] = v[j];
+    } else {
+        for (int j = 0; j < 8; j++) {
+            const int e = tid + 256 * j;
+            S[(e / TK) * LDK + e % TK] = v[j];
         }
     }
 }
@@ -90,8 +116,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     int64_t sbk, int64_t sbn, float *__restrict__ C, int64_t ldc, uint16_t *__restrict__ Cb,
     const float *__restrict__ bias, int32_t relu, int64_t M, int64_t N, int64_t K, int64_t kc,
     float *__restrict__ ws) {
-    __shared__ __align__(16) uint16_t As[TM * LDK];
-    __shared__ __align__(16) uint16_t Bs[TN * LDK];
+    // operand tiles during the k loop, the result tile after it
+    __shared__ __align__(16) unsigned char smem[TM * CLD * 4];
+    uint16_t *As = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *Bs = As + TM * LDK;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
     const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
@@ -101,11 +129,22 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     f32x4 acc[2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // op(B)(k, n): rows of the staged tile are n, its k stride sbk.  The
+    // next k step's tiles are loaded into registers while this one's
+    // products run.
+    u16x8 va, vb;
+    if (kbeg < kend) {
+        va = load_tile(A, sam, sak, m0, kbeg, M, kend, tid);
+        vb = load_tile(B, sbn, sbk, n0, kbeg, N, kend, tid);
+    }
     for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
-        stage(As, A, sam, sak, m0, k0, M, kend, tid);
-        // op(B)(k, n): rows of the staged tile are n, its k stride sbk
-        stage(Bs, B, sbn, sbk, n0, k0, N, kend, tid);
+        store_tile(As, va, sam, sak, tid);
+        store_tile(Bs, vb, sbn, sbk, tid);
         __syncthreads();
+        if (k0 + TK < kend) {
+            va = load_tile(A, sam, sak, m0, k0 + TK, M, kend, tid);
+            vb = load_tile(B, sbn, sbk, n0, k0 + TK, N, kend, tid);
+        }
         bf16x8 a[2], b[2];
         for (int i = 0; i < 2; i++) {
             a[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm + 16 * i + r16) * LDK + kq]);
@@ -116,22 +155,45 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+    // the tile through LDS, so that each thread writes 16 consecutive
+    // columns of one row (vector stores where the row lies inside C)
+    float *Cs = reinterpret_cast<float *>(smem);
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++)
-            for (int q = 0; q < 4; q++) {
-                const int64_t m = m0 + wm + 16 * i + 4 * (lane >> 4) + q;
-                const int64_t n = n0 + wn + 16 * j + r16;
-                if (m >= M || n >= N) continue;
-                float v = acc[i][j][q];
-                if (ws) {
-                    ws[((int64_t)blockIdx.z * M + m) * N + n] = v;
-                    continue;
-                }
-                if (bias) v += bias[n];
-                if (relu && !(v > 0.f)) v = 0.f;
-                C[m * ldc + n] = v;
-                if (Cb) Cb[m * ldc + n] = f32_to_bf16_rne(v);
+            for (int q = 0; q < 4; q++)
+                Cs[(wm + 16 * i + 4 * (lane >> 4) + q) * CLD + wn + 16 * j + r16] = acc[i][j][q];
+    __syncthreads();
+    const int row = tid >> 2, c0 = (tid & 3) * 16;
+    const int64_t m = m0 + row;
+    if (m >= M) return;
+    float *out = ws ? ws + ((int64_t)blockIdx.z * M + m) * N : C + m * ldc;
+    const bool vec = (((uintptr_t)out & 15) == 0) && (ws || !Cb || ((uintptr_t)(Cb + m * ldc) & 7) == 0);
+    for (int c = c0; c < c0 + 16; c += 4) {
+        const int64_t n = n0 + c;
+        if (n >= N) break;
+        f32x4 v = *reinterpret_cast<const f32x4 *>(&Cs[row * CLD + c]);
+        if (!ws) {
+            for (int e = 0; e < 4; e++) {
+                if (bias && n + e < N) v[e] += bias[n + e];
+                if (relu && !(v[e] > 0.f)) v[e] = 0.f;
             }
+        }
+        if (vec && n + 4 <= N) {
+            *reinterpret_cast<f32x4 *>(&out[n]) = v;
+            if (!ws && Cb) {
+                uint16_t b4[4] = {f32_to_bf16_rne(v[0]), f32_to_bf16_rne(v[1]),
+                                  f32_to_bf16_rne(v[2]), f32_to_bf16_rne(v[3])};
+                uint64_t packed;
+                __builtin_memcpy(&packed, b4, 8);
+                *reinterpret_cast<uint64_t *>(&Cb[m * ldc + n]) = packed;
+            }
+        } else {
+            for (int e = 0; e < 4 && n + e < N; e++) {
+                out[n + e] = v[e];
+                if (!ws && Cb) Cb[m * ldc + n + e] = f32_to_bf16_rne(v[e]);
+            }
+        }
+    }
 }
 
 /* Sum of the `splits` partial tiles in split order, then the epilogue. */
