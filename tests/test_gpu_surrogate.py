@@ -210,3 +210,27 @@ def test_batched_screened_scan_uneven_error_seeds():
         res['tracked_full_fraction'], [round(e['min_diff'], 3) for e in errs]), flush=True)
     for l, e in enumerate(errs):
         assert abs(e['min_diff']) <= 2 * 20.0 / n_r[l] + 1e-9, (l, e)
+
+
+def test_fit_graph_replays_equal_eager_steps():
+    """The training step captured once in a HIP graph and replayed per
+    minibatch (with the minibatch copies, a short last minibatch run eagerly
+    and the per-epoch loss read back between replays) gives the eager fit's
+    weights, biases and losses bit for bit, censored targets included."""
+    import torch
+    from xtrack_amd import surrogate as S
+    print('[fit graph]', flush=True)
+    dev = torch.device('cuda:0')
+    g = torch.Generator(device='cpu').manual_seed(3)
+    x = torch.randn(3000, 16, generator=g).to(dev)
+    y = (x[:, :4].sum(dim=1) + 0.1 * torch.randn(3000, generator=g).to(dev)).contiguous()
+    cens = (torch.rand(3000, generator=g) < 0.3).to(dev)
+    nets, hists = [], []
+    for graph in (False, True):
+        net = S.MLP(device=dev, seed=0)
+        hists.append(net.fit(x, y, epochs=4, batch=1024, seed=0, censored=cens, graph=graph))
+        nets.append(net)
+    assert hists[0] == hists[1], hists
+    for a, b in zip(nets[0].W + nets[0].b, nets[1].W + nets[1].b):
+        assert torch.equal(a, b)
+    assert hists[0][-1] < hists[0][0]

@@ -75,6 +75,26 @@ def gemm(a, b, trans_a=False, trans_b=False, bias=None, relu=False, want_bf16=Fa
     return c, cb
 
 
+def _small_sum(a):
+    """Sum over the first axis in reductions of at most 128 terms each
+    (rows in groups of 128, then the group sums), so that every reduction
+    kernel works within one block and leaves no global partials: the same
+    kernels run eagerly and inside a captured HIP graph."""
+    torch = _t()
+    n = a.shape[0]
+    g = -(-n // 128)
+    if g * 128 != n:
+        a = torch.cat([a, a.new_zeros((g * 128 - n,) + tuple(a.shape[1:]))])
+    part = a.reshape((g, 128) + tuple(a.shape[1:])).sum(dim=1)
+    while part.shape[0] > 128:
+        g2 = -(-part.shape[0] // 128)
+        if g2 * 128 != part.shape[0]:
+            part = torch.cat([part, part.new_zeros((g2 * 128 - part.shape[0],) +
+                                                   tuple(part.shape[1:]))])
+        part = part.reshape((g2, 128) + tuple(part.shape[1:])).sum(dim=1)
+    return part.sum(dim=0)
+
+
 class MLP:
     """16 -> 256 -> 256 -> 256 -> 1 with ReLU; FP32 master weights, BF16
     products on the matrix cores."""
@@ -121,56 +141,95 @@ class MLP:
         for li in range(n - 1, -1, -1):
             dzb = to_bf16(dz)
             gW[li], _ = gemm(acts[li], dzb, trans_a=True)       # h^T dz
-            gb[li] = dz.sum(dim=0)
+            gb[li] = _small_sum(dz)
             if li > 0:
                 dh, _ = gemm(dzb, to_bf16(self.W[li]), trans_b=True)   # dz W^T
                 dz = dh * (zs[li - 1] > 0).to(dh.dtype)
         return gW, gb
 
     def adam_step(self, gW, gb, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        """One Adam update of the weights and biases in place.  The step
+        count and the bias corrections live on the device, so that the update
+        can be captured in a HIP graph and replayed."""
         torch = _t()
         if self._adam is None:
-            self._adam = {'t': 0, 'm': [torch.zeros_like(p) for p in self.W + self.b],
+            self._adam = {'t': torch.zeros((), dtype=torch.float32, device=self.device),
+                          'm': [torch.zeros_like(p) for p in self.W + self.b],
                           'v': [torch.zeros_like(p) for p in self.W + self.b]}
         st = self._adam
-        st['t'] += 1
+        st['t'].add_(1.0)
+        bc1 = 1.0 - torch.pow(beta1, st['t'])
+        bc2 = 1.0 - torch.pow(beta2, st['t'])
         params, grads = self.W + self.b, gW + gb
         for k, (p, g) in enumerate(zip(params, grads)):
             st['m'][k].mul_(beta1).add_(g, alpha=1 - beta1)
             st['v'][k].mul_(beta2).addcmul_(g, g, value=1 - beta2)
-            mh = st['m'][k] / (1 - beta1 ** st['t'])
-            vh = st['v'][k] / (1 - beta2 ** st['t'])
+            mh = st['m'][k] / bc1
+            vh = st['v'][k] / bc2
             p.sub_(lr * mh / (vh.sqrt() + eps))
 
-    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0, censored=None):
+    def _train_step(self, xb, yb, cb, lr):
+        """Forward, backward and Adam on one minibatch; returns the sum of
+        squared errors (device scalar).  cb: None or the censored flags."""
+        torch = _t()
+        out = self.forward(xb, keep=True)
+        err = out - yb
+        if cb is not None:
+            err = torch.where(cb & (err > 0), 0.0, err)
+        gW, gb = self.backward(2.0 * err / len(xb))
+        self.adam_step(gW, gb, lr=lr)
+        return _small_sum(err * err)
+
+    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0, censored=None, graph=False):
         """Mean-squared error on standardised inputs and targets; x (N, 16),
         y (N,) torch FP32 on the device.  censored: None, or a bool (N,)
         tensor marking targets that are lower bounds (particles alive at the
         last tracked turn): their error counts only where the output lies
-        below the target, so the network may place them beyond it.  Returns
-        the loss per epoch."""
+        below the target, so the network may place them beyond it.  With
+        graph=True the step on a full minibatch is captured once in a HIP
+        graph (torch.cuda.CUDAGraph) and replayed with each minibatch copied
+        into its input buffers, the same kernels in the same order as the
+        eager step (graph=False); a short last minibatch runs eagerly.
+        Returns the loss per epoch."""
         torch = _t()
         self.x_mean = x.mean(dim=0)
         self.x_std = x.std(dim=0).clamp_min(1e-12)
         self.y_mean, self.y_std = float(y.mean()), float(y.std().clamp_min(1e-12))
-        xn = (x - self.x_mean) / self.x_std
-        yn = (y - self.y_mean) / self.y_std
+        xn = ((x - self.x_mean) / self.x_std).contiguous()
+        yn = ((y - self.y_mean) / self.y_std).contiguous()
         cens = None if censored is None else censored.to(self.device, torch.bool)
         g = torch.Generator(device='cpu').manual_seed(seed)
         hist = []
         n = len(xn)
+        B = min(int(batch), n)
+        gr = None
+        sx = sy = sc = sloss = None
         for _ in range(epochs):
             perm = torch.randperm(n, generator=g).to(self.device)
             tot = torch.zeros((), dtype=torch.float32, device=self.device)
-            for s in range(0, n, batch):
-                idx = perm[s:s + batch]
-                out = self.forward(xn[idx], keep=True)
-                err = out - yn[idx]
-                if cens is not None:
-                    err = torch.where(cens[idx] & (err > 0), torch.zeros_like(err), err)
-                gW, gb = self.backward(2.0 * err / len(idx))
-                self.adam_step(gW, gb, lr=lr)
-                tot += (err * err).sum()
+            for s in range(0, n, B):
+                idx = perm[s:s + B]
+                full = len(idx) == B
+                if graph and full and gr is not None:
+                    sx.copy_(xn[idx])
+                    sy.copy_(yn[idx])
+                    if sc is not None:
+                        sc.copy_(cens[idx])
+                    gr.replay()
+                    tot += sloss
+                    continue
+                tot += self._train_step(xn[idx], yn[idx],
+                                        None if cens is None else cens[idx], lr)
+                if graph and full and gr is None:
+                    # the eager step above allocated the optimiser state and
+                    # the product workspace; capture the next one
+                    sx = torch.empty_like(xn[:B])
+                    sy = torch.empty_like(yn[:B])
+                    sc = None if cens is None else torch.empty_like(cens[:B])
+                    torch.cuda.synchronize(self.device)
+                    gr = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(gr):
+                        sloss = self._train_step(sx, sy, sc, lr)
             hist.append(float(tot) / n)
         return hist
 
