@@ -277,17 +277,24 @@ def _compare(got, ref):
             'bit_identical': bits, 'max_rel_err': max(err.values()), 'rel_err': err}
 
 
-def _oracle_track(line_fn, parts, turns, math, threads, snapshots=()):
+def _oracle_track(line_fn, parts, turns, math, threads, snapshots=(), log_every=None,
+                  what=''):
     """Oracle track of `parts`; sorted copies after each turn count in
-    `snapshots` and at the end."""
+    `snapshots` and at the end; a progress line every `log_every` turns."""
     line = line_fn()
     line.build_tracker(backend=_oracle_backend(math, threads))
     p = parts.copy()
     out, done = {}, 0
-    for t in sorted(set(snapshots) | {turns}):
+    stops = set(snapshots) | {turns}
+    if log_every:
+        stops |= set(range(log_every, turns, log_every))
+    for t in sorted(stops):
         line.track(p, num_turns=t - done)
         done = t
-        out[t] = p.sorted_by_id()
+        if t in snapshots or t == turns:
+            out[t] = p.sorted_by_id()
+        if log_every:
+            log(f'oracle ({math}): {what}, turn {t} of {turns}')
     return out
 
 
@@ -366,7 +373,10 @@ def parity_and_cpu_legs(args, make_line, parts_gpu, init, turns_done):
     o_thin = {m: _oracle_track(lattices.thin_fodo_ring, thin_beam, 1000, m, threads, (1,))
               for m in ('xm', 'libm')}
     log('oracle: thin FODO workload')
-    o_col = {m: _oracle_track(make_line, csamp, 1000, m, threads, (1,)) for m in ('xm', 'libm')}
+    # 64 particles: one host thread is faster than the OpenMP chunking of
+    # the restated CPU tracker over 16 threads (synchronisation per element)
+    o_col = {m: _oracle_track(make_line, csamp, 1000, m, 1, (1,), log_every=250,
+                              what='collider sample') for m in ('xm', 'libm')}
     log('oracle: collider sample')
     cpu = cpu_baseline(make_line, init, lattices.thin_fodo_ring, thin_beam, args)
     log('CPU baseline legs')

@@ -180,7 +180,7 @@ class MLP:
         self.adam_step(gW, gb, lr=lr)
         return _small_sum(err * err)
 
-    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0, censored=None, graph=False):
+    def fit(self, x, y, epochs=200, batch=1024, lr=2e-3, seed=0, censored=None, graph=True):
         """Mean-squared error on standardised inputs and targets; x (N, 16),
         y (N,) torch FP32 on the device.  censored: None, or a bool (N,)
         tensor marking targets that are lower bounds (particles alive at the
