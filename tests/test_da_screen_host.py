@@ -128,3 +128,47 @@ def test_band_uses_the_ensemble_range():
     lo[20] = 100.0                       # one member predicts a loss at slot 20
     band = S.boundary_band(gs, guess, rest, T, 1.0, 0.26, guess_lo=lo, guess_hi=hi)
     assert set(band.tolist()) == {k for k in rest if abs(gs.r_slot[k] - gs.r_slot[20]) <= 0.26}
+
+
+def test_small_sum_equals_the_plain_sum():
+    import torch
+    g = torch.Generator().manual_seed(0)
+    for shape in ((1024, 256), (1000, 7), (5,), (1, 3), (130,), (20000, 2), (128,)):
+        a = torch.randn(shape, generator=g, dtype=torch.float64)
+        assert torch.allclose(S._small_sum(a), a.sum(dim=0), rtol=0, atol=1e-12)
+
+
+def test_features_tune_diffusion_from_accumulators():
+    """Synthetic accumulators of three particles over a 10-turn window:
+    constant phase advance 0.31 turn (no diffusion), advance 0.30 in the
+    first half and 0.32 in the second (diffusion 0.02), and a particle lost
+    after 4 turns (no second half: diffusion input 0)."""
+    import torch
+    window = 10
+    acc = torch.zeros((17, 3), dtype=torch.float64)
+    two_pi = 2 * np.pi
+    counts = [10, 10, 4]
+    adv1 = [0.31, 0.30, 0.25]
+    adv2 = [0.31, 0.32, 0.25]
+    for i in range(3):
+        n = counts[i]
+        acc[0, i] = n
+        # advances at turns 1..n-1; second half: turns >= 5
+        turns = np.arange(1, n)
+        ph = np.where(turns >= 5, adv2[i], adv1[i]) * two_pi
+        acc[7, i] = acc[8, i] = ph.sum()
+        acc[15, i] = acc[16, i] = ph[turns >= 5].sum()
+        acc[1, i] = acc[4, i] = n * 2.0          # mean action 2
+        acc[2, i] = acc[5, i] = n * 4.0
+        acc[3, i] = acc[6, i] = 2.0
+        acc[13, i] = acc[14, i] = 2.0
+    f = S.features_from_acc(acc, np.array([1.0, 1.0, 1.0]), np.zeros(3), np.zeros(3), window)
+    f = f.numpy().astype(np.float64)
+    # tunes: mean advance per turn over the window
+    assert np.allclose(f[:, 11], [0.31, (4 * 0.30 + 5 * 0.32) / 9, 0.25], atol=1e-6)
+    # log10 tune diffusion: 0 -> -12 (floor), 0.02, and 0 for the lost one
+    assert f[0, 14] < -6 and abs(f[1, 14] - np.log10(0.02)) < 1e-4 and f[2, 14] == 0.0
+    assert np.array_equal(f[:, 14], f[:, 15])
+    # alive fraction, action features
+    assert np.allclose(f[:, 13], [1.0, 1.0, 0.4])
+    assert np.allclose(f[:, 5], 0.0) and np.allclose(f[:, 7], 0.0)
