@@ -645,6 +645,8 @@ def main_da(args):
         out_scans[pred] = {
             'wall_s': round(wall, 2), 'speedup_vs_plain': plain['time'] / wall,
             'tracked_full_fraction': round(res['tracked_full_fraction'], 4),
+            'band_particles': int(len(res['band'])),
+            'band_stopped_above_a_loss': res['band_stopped_above_a_loss'],
             'parts_s': {k: round(v, 2) for k, v in res['time'].items()},
             'da_error_sigma_per_seed': [{k: round(v, 3) for k, v in e.items()} for e in errs],
             'da_error_sigma_max': max(e['max_abs'] for e in errs),

@@ -410,14 +410,17 @@ def _starts_of(slots_sorted, starts):
     return np.searchsorted(slots_sorted, starts).astype(np.int64)
 
 
-def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=None):
+def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=None,
+                 prune=None):
     """Track the grid particles `slots` (sorted numpy slot indices) of
     GridSet gs for num_turns turns on BatchTracker bt, compacting the lost
     ones out every `block` turns (all turns in one launch when None), and
     write the results back into gs.particles.  features: None, or
     (inv_w4, co) to gather the feature accumulators (with block None);
     returns them for the slots, else None.  progress: None, or called as
-    progress(turns done, particles still tracked) after each block."""
+    progress(turns done, particles still tracked) after each block.
+    prune: None, or a function of the slots still alive after a block that
+    returns those to keep tracking (the others stop where they are)."""
     torch = _t()
     dev = gs.device
     slots = np.asarray(slots, dtype=np.int64)
@@ -438,6 +441,8 @@ def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=N
         if done < num_turns:
             alive = (sub._arrays['state'] > 0).cpu().numpy()
             slots = slots[alive]
+            if prune is not None and len(slots):
+                slots = prune(slots)
         if progress is not None:
             progress(done, len(slots))
     torch.cuda.synchronize(dev)
@@ -561,7 +566,7 @@ def boundary_band(gs, guess, rest, num_turns, band_factor, band_sigma, lost_belo
 def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
                      band_factor=3.0, band_sigma=0.5, epochs=150, seed=0, net=None,
                      predictor='mlp', block=256, normalisation=None, censor_margin=None,
-                     ensemble=4, progress=None):
+                     ensemble=4, prune_band=True, progress=None):
     """Dynamic-aperture scan of every lattice of GridSet gs screened by the
     survival surrogate, on BatchTracker bt (the lattices of gs, in order):
 
@@ -584,7 +589,10 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
        factor band_factor of num_turns, or whose amplitude lies within
        band_sigma beam sigmas of their angle's predicted aperture (the
        smallest amplitude that any network predicts lost), are tracked to
-       num_turns (the boundary band);
+       num_turns (the boundary band); with prune_band, a band particle
+       that finds itself above a loss of its group (one of the band lost
+       earlier in the pass) stops at the next compaction, since it can no
+       longer change the aperture;
     5. every other particle keeps its prediction (the ensemble's median):
        alive if predicted to survive at least half of num_turns, else lost
        at its predicted turn.
@@ -668,23 +676,37 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     relevant = rest[below_first_loss(gs, exact & (surv < num_turns))[rest]]
     band = boundary_band(gs, guess, relevant, num_turns, band_factor, band_sigma,
                          guess_lo=lo, guess_hi=hi)
+    # a band particle above a loss of its group, known or found while the
+    # band is tracked, can no longer change the group's aperture: it stops
+    # at the next compaction
+    def _above_no_loss(slots):
+        lost_now = gs.particles._arrays['state'].cpu().numpy() <= 0
+        return slots[below_first_loss(gs, lost_now)[slots]]
+
     t0 = time.perf_counter()
     track_subset(bt, gs, band, num_turns - feature_turns, block=block,
                  progress=None if progress is None else
-                 (lambda d, n: progress('band', feature_turns + d, n)))
+                 (lambda d, n: progress('band', feature_turns + d, n)),
+                 prune=_above_no_loss if prune_band else None)
     times['band_tracking'] = time.perf_counter() - t0
     surv_t = survival_of(gs, num_turns)
-    surv[band] = surv_t[band]
-    exact[band] = True
+    reached = gs.particles._arrays['at_turn'].cpu().numpy()
+    st_now = gs.particles._arrays['state'].cpu().numpy()
+    full = band[(st_now[band] <= 0) | (reached[band] >= num_turns)]
+    surv[full] = surv_t[full]
+    exact[full] = True
     # 5. the rest keeps its prediction (the ensemble's median): alive when
     # predicted to survive half the turns or more
+    # (a band particle stopped early keeps at least the turn it reached)
     left = np.flatnonzero(~exact)
     p_left = guess[left]
+    low = np.maximum(feature_turns, np.minimum(reached[left], num_turns - 1))
     surv[left] = np.where(p_left >= 0.5 * num_turns, num_turns,
-                          np.clip(p_left, feature_turns, num_turns - 1)).astype(np.int64)
+                          np.clip(p_left, low, num_turns - 1)).astype(np.int64)
     times['total'] = time.perf_counter() - t_all
     return {'survival': surv, 'exact': exact, 'labelled': lab, 'band': band,
-            'tracked_full_fraction': (len(lab) + len(band)) / n_all,
+            'band_stopped_above_a_loss': int(len(band) - len(full)),
+            'tracked_full_fraction': (len(lab) + len(full)) / n_all,
             'da': da_per_line(gs, surv, num_turns), 'loss_history': hist, 'time': times,
             'net': nets[0] if nets else None, 'nets': nets}
 
