@@ -505,9 +505,11 @@ def main_mlp(args):
     BF16 operands, FP32 accumulation, fused bias, ReLU and BF16 rounding of
     the output): the hidden-layer product of inference over a DA grid,
     (65,536 x 256) x (256 x 256), timed with HIP events on the launch
-    stream over `--steps` launches, and one training step of the
+    stream over `--steps` launches, and the training step of the
     16-256-256-256-1 network at the minibatch of the fit (1,024 samples:
-    forward, backward and Adam).  The product moves 32 MiB of BF16 input,
+    forward, backward and Adam), launched kernel by kernel and replayed from
+    a captured HIP graph (wall time per step over 100 steps).  The product
+    moves 32 MiB of BF16 input,
     64 MiB of FP32 and 32 MiB of BF16 output per launch for 8.6 GFLOP
     (64 flop per byte, below the 312 flop per byte where MI355X's BF16
     matrix peak and its HBM bandwidth meet), so it is bound by HBM."""
@@ -537,18 +539,20 @@ def main_mlp(args):
     ms_mean = float(np.mean(ms))
     flop = 2.0 * M * N * K
     bytes_ = 2.0 * M * K + 2.0 * K * N + 4.0 * N + (4.0 + 2.0) * M * N
-    # one training step at the fit's minibatch
-    net = S.MLP(device=dev, seed=0)
-    xb = torch.randn(1024, 16, generator=g).to(dev)
-    yb = torch.randn(1024, generator=g).to(dev)
-    net.fit(xb, yb, epochs=2, batch=1024, seed=0)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    net.fit(xb, yb, epochs=10, batch=1024, seed=0)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    train_ms = e0.elapsed_time(e1) / 10
+    # training steps at the fit's minibatch (1,024 of 4,096 samples), launched
+    # one by one and replayed from the captured graph (one eager step and
+    # the capture included, over 100 steps)
+    xb = torch.randn(4096, 16, generator=g).to(dev)
+    yb = torch.randn(4096, generator=g).to(dev)
+    train_ms = {}
+    for mode in ('eager', 'graph'):
+        net = S.MLP(device=dev, seed=0)
+        net.fit(xb, yb, epochs=1, batch=1024, seed=0, graph=(mode == 'graph'))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        net.fit(xb, yb, epochs=25, batch=1024, seed=0, graph=(mode == 'graph'))
+        torch.cuda.synchronize()
+        train_ms[mode] = round((time.perf_counter() - t1) / 100 * 1e3, 4)
     achieved = bytes_ / (ms_mean * 1e-3) / 1e9
     out = {
         'metric': 'survival-network hidden-layer product, HBM GB/s', 'value': achieved,
@@ -560,7 +564,7 @@ def main_mlp(args):
                                f'output (inference of the 16-256-256-256-1 network over '
                                f'{M} particles, hidden layer)',
                    'kernel_ms': [round(v, 4) for v in ms], 'tflops': flop / (ms_mean * 1e-3) / 1e12,
-                   'training_step_ms_batch_1024': round(train_ms, 3)},
+                   'training_step_ms_batch_1024': train_ms},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': 8000.0, 'unit': 'GB/s',
                      'frac': achieved / 8000.0, 'traffic': None,
                      'algorithmic_bytes_per_launch': bytes_, 'flop_per_launch': flop,
