@@ -273,7 +273,8 @@ def _put(p, idx, sub):
             vv[idx] = sub._arrays[nn]
 
 
-def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, starts=None):
+def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, starts=None,
+                   acc=None, window=None):
     """Track `particles` (a device particle set) for num_turns whole turns
     with the feature accumulators updated on the device at the start of every
     turn, between one-turn tracking launches (xtm_track_line_features /
@@ -282,15 +283,23 @@ def track_features(line_or_batch, particles, num_turns, inv_w4, co, sx, sy, star
     device, row k = accumulator k of every particle slot.
 
     line_or_batch: a Line with a device tracker, or a BatchTracker together
-    with `starts` (the variants' particle ranges in `particles`)."""
+    with `starts` (the variants' particle ranges in `particles`).  acc and
+    window: accumulators to continue (one column per particle slot) and the
+    recorded turn window (start, stop); by default fresh accumulators and
+    the window of these num_turns turns."""
     torch = _t()
     lib = _native.load()
     n = particles._capacity
-    acc = torch.zeros((_native.XTM_FEAT_N, n), dtype=torch.float64,
-                      device=particles._arrays['x'].device)
-    t0 = int(particles._arrays['at_turn'].min().item()) if n else 0
-    fst = _native.make_features_struct(acc, n, t0, t0 + num_turns, np.asarray(inv_w4),
-                                       co, (1.0 / sx, 1.0 / sy))
+    if acc is None:
+        acc = torch.zeros((_native.XTM_FEAT_N, n), dtype=torch.float64,
+                          device=particles._arrays['x'].device)
+    elif tuple(acc.shape) != (_native.XTM_FEAT_N, n) or not acc.is_contiguous():
+        raise ValueError('acc must be a contiguous (XTM_FEAT_N, capacity) tensor')
+    if window is None:
+        t0 = int(particles._arrays['at_turn'].min().item()) if n else 0
+        window = (t0, t0 + num_turns)
+    fst = _native.make_features_struct(acc, n, int(window[0]), int(window[1]),
+                                       np.asarray(inv_w4), co, (1.0 / sx, 1.0 / sy))
     pst = _native.make_particles_struct(particles)
     if starts is None:
         tr = line_or_batch.tracker
@@ -417,14 +426,21 @@ def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=N
     ones out every `block` turns (all turns in one launch when None), and
     write the results back into gs.particles.  features: None, or
     (inv_w4, co) to gather the feature accumulators (with block None);
-    returns them for the slots, else None.  progress: None, or called as
-    progress(turns done, particles still tracked) after each block.
-    prune: None, or a function of the slots still alive after a block that
-    returns those to keep tracking (the others stop where they are)."""
+    returns them for the slots (columns in the order of `slots`, over the
+    window of these num_turns turns, continued across blocks), else None.
+    progress: None, or called as progress(turns done, particles still
+    tracked) after each block.  prune: None, or a function of the slots
+    still alive after a block that returns those to keep tracking (the
+    others stop where they are)."""
     torch = _t()
     dev = gs.device
     slots = np.asarray(slots, dtype=np.int64)
     done, acc = 0, None
+    if features is not None:
+        acc = torch.zeros((_native.XTM_FEAT_N, len(slots)), dtype=torch.float64, device=dev)
+        col0 = slots.copy()
+        t0 = int(gs.particles._arrays['at_turn'][torch.as_tensor(slots, device=dev)]
+                 .min().item()) if len(slots) else 0
     while done < num_turns and len(slots):
         n = num_turns - done if block is None else min(block, num_turns - done)
         idx = torch.as_tensor(slots, device=dev)
@@ -432,7 +448,11 @@ def track_subset(bt, gs, slots, num_turns, block=None, features=None, progress=N
         lay = _starts_of(slots, gs.starts)
         if features is not None:
             inv_w4, co = features
-            acc = track_features(bt, sub, n, inv_w4, co, gs.sigma[0], gs.sigma[1], starts=lay)
+            cols = torch.as_tensor(np.searchsorted(col0, slots), device=dev)
+            acc_sub = acc[:, cols].contiguous()
+            track_features(bt, sub, n, inv_w4, co, gs.sigma[0], gs.sigma[1], starts=lay,
+                           acc=acc_sub, window=(t0, t0 + num_turns))
+            acc[:, cols] = acc_sub
         else:
             bt._set_layout(lay, bt.config or (256, 4))
             bt._launch(sub, n)
@@ -571,7 +591,11 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     survival surrogate, on BatchTracker bt (the lattices of gs, in order):
 
     1. every grid particle is tracked for feature_turns turns with the
-       feature accumulators gathered in the kernel;
+       feature accumulators gathered in the kernel (with prune_band, a
+       particle stops at the next 8-turn compaction once a particle below
+       it in its group is lost: it can no longer change the aperture and is
+       neither labelled nor banded; its survival is then predicted from
+       the features of the turns it made);
     2. a random label_fraction of the survivors of each lattice that lie
        below the first loss of their (angle, delta) group (only those can
        change the aperture) is tracked to num_turns: their survival turns
@@ -610,7 +634,16 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     # 1. feature turns
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    acc = track_subset(bt, gs, np.arange(n_all), feature_turns, features=normalisation)
+    # a particle above a loss of its (lattice, angle, delta) group can no
+    # longer change the group's aperture: it stops at the next compaction
+    # (the feature pass compacts every 8 turns, the band every `block`)
+    def _above_no_loss(slots):
+        lost_now = gs.particles._arrays['state'].cpu().numpy() <= 0
+        return slots[below_first_loss(gs, lost_now)[slots]]
+
+    acc = track_subset(bt, gs, np.arange(n_all), feature_turns, features=normalisation,
+                       block=8 if prune_band else None,
+                       prune=_above_no_loss if prune_band else None)
     feat = features_from_acc(acc, gs.x_norm, gs.y_norm, gs.delta_slot, feature_turns)
     torch.cuda.synchronize(dev)
     times['features'] = time.perf_counter() - t0
@@ -676,13 +709,6 @@ def screened_da_scan(bt, gs, num_turns, feature_turns=256, label_fraction=0.02,
     relevant = rest[below_first_loss(gs, exact & (surv < num_turns))[rest]]
     band = boundary_band(gs, guess, relevant, num_turns, band_factor, band_sigma,
                          guess_lo=lo, guess_hi=hi)
-    # a band particle above a loss of its group, known or found while the
-    # band is tracked, can no longer change the group's aperture: it stops
-    # at the next compaction
-    def _above_no_loss(slots):
-        lost_now = gs.particles._arrays['state'].cpu().numpy() <= 0
-        return slots[below_first_loss(gs, lost_now)[slots]]
-
     t0 = time.perf_counter()
     track_subset(bt, gs, band, num_turns - feature_turns, block=block,
                  progress=None if progress is None else
