@@ -119,8 +119,10 @@ def parse():
     ap.add_argument('--batch-graph', action='store_true',
                     help='replay each step from a captured HIP graph')
     ap.add_argument('--batch-compile', action='store_true',
-                    help='compile the variants into one kernel (element structures with '
-                         'constants read from each line) instead of the interpreter')
+                    help='(the default) compile the variants into one kernel (element '
+                         'structures with constants read from each line)')
+    ap.add_argument('--batch-interpreter', action='store_true',
+                    help='track the batch with the element-program interpreter instead')
     ap.add_argument('--batch-waves-per-eu', type=int, default=0,
                     help='occupancy target of the compiled batch code (0: library default)')
     return ap.parse_args()
@@ -418,7 +420,7 @@ def batch_leg(args, steps, kernels=None):
     import xtrack_amd as xt
     from xtrack_amd import lattices, _native
     if kernels is None:
-        kernels = ('compiled',) if args.batch_compile else ('interpreter',)
+        kernels = ('interpreter',) if args.batch_interpreter else ('compiled',)
     rng = np.random.default_rng(12345)
     nv = args.batch_variants
     counts = np.exp(rng.uniform(np.log(1000), np.log(args.batch_max_particles), nv))
