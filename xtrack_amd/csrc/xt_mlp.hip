@@ -17,7 +17,8 @@
 // multiplies its 32 x 32 quarter as 2 x 2 tiles of gfx950's
 // v_mfma_f32_16x16x32_bf16.  The result tile goes back through LDS so
 // that each thread writes 16 consecutive columns of one row with vector
-// stores (FP32, and the BF16 copy).  Operand layout of that instruction (64 lanes,
+// stores (FP32, and the BF16 copy).  Block ids map to tiles so that the
+// column tiles of one row tile run on one XCD (shared op(A) rows in one L2).  Operand layout of that instruction (64 lanes,
 // lane l): A holds row l % 16 and the eight consecutive k = 8 (l / 16) + j;
 // B holds column l % 16 and the same k; the accumulator holds column l % 16
 // and rows 4 (l / 16) + j, j < 4.
@@ -121,7 +122,18 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     uint16_t *As = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Bs = As + TM * LDK;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+    // Workgroups go to the eight XCDs round-robin by linear id, and each XCD
+    // has its own L2: map the ids so that the column tiles of one row tile
+    // (which read the same op(A) rows) run on one XCD.  Row tile t goes to
+    // XCD t % 8; a bijection when the row-tile count is a multiple of 8.
+    int64_t ty = blockIdx.y, tx = blockIdx.x;
+    if ((gridDim.y & 7) == 0) {
+        const int64_t lin = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+        const int64_t xcd = lin & 7, local = lin >> 3;
+        tx = local % gridDim.x;
+        ty = xcd + 8 * (local / gridDim.x);
+    }
+    const int64_t m0 = ty * TM, n0 = tx * TN;
     const int wm = 32 * (w >> 1), wn = 32 * (w & 1);
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
     const int64_t kbeg = (int64_t)blockIdx.z * kc;
