@@ -61,8 +61,8 @@ def test_noalloc_gemm_equals_allocating_gemm():
 
 def test_noalloc_gemm_argument_errors():
     """An invalid call is refused with a message (ldc < N here); the split
-    chosen for any valid product keeps tiles x splits <= 256, so its
-    partials (at most 4096 floats a tile) always fit the array."""
+    chosen for any valid product keeps splits x M x N within the array, so
+    its partials always fit."""
     lib = _native.load()
     import torch
     a, b, _ = _operands(128, 128, 64, False, False, 4)
